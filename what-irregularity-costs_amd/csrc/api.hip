@@ -1,0 +1,723 @@
+// Host side of the gfx950 TSDF arm: volume lifetime, stream discipline and the
+// extern "C" entry points declared in include/osn_tsdf_hip.h.
+//
+// Mirrors the reference device ABI's behaviour (cpp/src/c_api_cuda.cu:151-246,
+// cpp/src/cuda_impl.cuh:50-111, cpp/src/volume_cuda.cu:343-478): one private
+// stream per volume, the library owns every device allocation, getters
+// synchronise, device failures are printed as "[osn_tsdf_hip] file:line" and
+// mark the volume invalid. Differences, by design: allocation runs as
+// begin -> insert -> assign-in-key-order (no index publication spin), and
+// extraction scratch is allocated once and grown, never per call.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/osn_tsdf_hip.h"
+#include "internal.hpp"
+
+using namespace osn_hip;
+
+static_assert(sizeof(OsnTsdfDeviceView) == 104, "OsnTsdfDeviceView must stay 104 bytes");
+
+namespace {
+
+constexpr uint32_t kScratchSlots = 1u << 20;  // reference types.hpp:30-33
+constexpr int kStages = 4;
+
+#define HIP_OK(expr)                                                                        \
+    do {                                                                                    \
+        const hipError_t err_ = (expr);                                                     \
+        if (err_ != hipSuccess) {                                                           \
+            std::fprintf(stderr, "[osn_tsdf_hip] %s:%d: %s -> %s\n", __FILE__, __LINE__,   \
+                         #expr, hipGetErrorString(err_));                                   \
+            return false;                                                                   \
+        }                                                                                   \
+    } while (0)
+
+uint32_t next_pow2(uint32_t v) {
+    if (v < 2) return 2;
+    --v;
+    v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
+    return v + 1;
+}
+
+size_t voxel_bytes(int32_t storage) {
+    switch (storage) {
+        case kStoreSums: return 20;
+        case kStoreF32: return 8;
+        case kStoreF16: return 4;
+        case kStoreF64: return 16;
+        default: return 0;
+    }
+}
+
+}  // namespace
+
+struct OsnTsdfHipVolume {
+    OsnTsdfHipConfig cfg{};
+    bool reference_shape = false;  // created by osn_tsdf_hip_create
+    bool ok = false;
+    hipStream_t stream = nullptr;
+    VolDev d{};
+    uint32_t table_alloc = 0;      // slots allocated (live table + scratch region)
+    int32_t lds_set = 1024;
+    int32_t update_threads = 256;
+    int32_t update_grid = 0;
+    // extraction scratch, grown on demand
+    int32_t* ex_vcount = nullptr;
+    int32_t* ex_voff = nullptr;
+    int32_t* ex_chunks = nullptr;
+    int32_t* ex_flag_count = nullptr;  // [0] = arrival counter, [1] = vertex count
+    int32_t ex_blocks_cap = 0;
+    float* stage_posnor = nullptr;
+    uint8_t* stage_rgb = nullptr;
+    int64_t stage_cap = 0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double ms[kStages] = {0, 0, 0, 0};
+    int64_t calls[kStages] = {0, 0, 0, 0};
+
+    ~OsnTsdfHipVolume();
+    bool alloc();
+    bool reset();
+    bool fail(const char* what, hipError_t e);
+    hipEvent_t take_event();
+    void stage_begin(int stage, hipEvent_t& a);
+    void stage_end(int stage, hipEvent_t a);
+    bool grow_extract(int32_t blocks);
+    bool grow_stage(int64_t verts, bool rgb);
+};
+
+bool OsnTsdfHipVolume::fail(const char* what, hipError_t e) {
+    std::fprintf(stderr, "[osn_tsdf_hip] %s -> %s\n", what, hipGetErrorString(e));
+    ok = false;
+    return false;
+}
+
+bool OsnTsdfHipVolume::alloc() {
+    const uint32_t cap = d.capacity;
+    HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    table_alloc = cap + (reference_shape ? kScratchSlots : 0u);
+    HIP_OK(hipMalloc(&d.table, (size_t)table_alloc * sizeof(HashEntry)));
+    HIP_OK(hipMalloc(&d.ctr, sizeof(Counters)));
+    HIP_OK(hipMalloc(&d.block_coord, (size_t)cfg.pool_capacity_blocks * 3 * sizeof(int32_t)));
+    const size_t n_vox = (size_t)cfg.pool_capacity_blocks * kBlockVoxels;
+    if (d.storage == kStoreSums) {
+        HIP_OK(hipMalloc(&d.tsdf, n_vox * sizeof(float)));
+        HIP_OK(hipMalloc(&d.weight, n_vox * sizeof(float)));
+        HIP_OK(hipMalloc(&d.r, n_vox * sizeof(float)));
+        HIP_OK(hipMalloc(&d.g, n_vox * sizeof(float)));
+        HIP_OK(hipMalloc(&d.b, n_vox * sizeof(float)));
+    } else {
+        HIP_OK(hipMalloc(&d.vox, n_vox * voxel_bytes(d.storage)));
+    }
+    HIP_OK(hipMalloc(&d.new_keys, (size_t)cap * sizeof(int64_t)));
+    HIP_OK(hipMalloc(&d.touched, (size_t)cap * sizeof(uint32_t)));
+    HIP_OK(hipMalloc(&d.sort_a, (size_t)cap * sizeof(int64_t)));
+    HIP_OK(hipMalloc(&d.sort_b, (size_t)cap * sizeof(int64_t)));
+    const size_t n_flags = (size_t)kMaxSortLevels * sort_flag_stride(cap);
+    HIP_OK(hipMalloc(&d.sort_flags, n_flags * sizeof(int32_t)));
+    HIP_OK(hipMemsetAsync(d.sort_flags, 0, n_flags * sizeof(int32_t), stream));
+    if (reference_shape) {
+        // the sentinel region past the live table (cpp/src/cuda_impl.cuh:61-67,
+        // 84-88): keys -2 can never satisfy a CAS against the empty key, so
+        // the region is write-once and reset() leaves it alone
+        std::vector<HashEntry> host(kScratchSlots, HashEntry{-2, -1, 0});
+        HIP_OK(hipMemcpyAsync(d.table + cap, host.data(), kScratchSlots * sizeof(HashEntry),
+                              hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    return reset();
+}
+
+bool OsnTsdfHipVolume::reset() {
+    // The table is refilled with the empty key (a zeroed table would read as
+    // occupied at block (0,0,0)); counters and voxel storage are zeroed.
+    HIP_OK(launch_fill_table(d.table, d.capacity, stream));
+    HIP_OK(hipMemsetAsync(d.ctr, 0, sizeof(Counters), stream));
+    const size_t n_vox = (size_t)cfg.pool_capacity_blocks * kBlockVoxels;
+    if (d.storage == kStoreSums) {
+        HIP_OK(hipMemsetAsync(d.tsdf, 0, n_vox * sizeof(float), stream));
+        HIP_OK(hipMemsetAsync(d.weight, 0, n_vox * sizeof(float), stream));
+        HIP_OK(hipMemsetAsync(d.r, 0, n_vox * sizeof(float), stream));
+        HIP_OK(hipMemsetAsync(d.g, 0, n_vox * sizeof(float), stream));
+        HIP_OK(hipMemsetAsync(d.b, 0, n_vox * sizeof(float), stream));
+    } else {
+        HIP_OK(hipMemsetAsync(d.vox, 0, n_vox * voxel_bytes(d.storage), stream));
+    }
+    HIP_OK(hipStreamSynchronize(stream));
+    ok = true;
+    return true;
+}
+
+OsnTsdfHipVolume::~OsnTsdfHipVolume() {
+    if (stream) hipStreamSynchronize(stream);
+    for (auto& p : ev_pending) {
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    for (auto e : ev_pool) hipEventDestroy(e);
+    void* ptrs[] = {d.table, d.ctr, d.block_coord, d.tsdf, d.weight, d.r, d.g, d.b, d.vox,
+                    d.new_keys, d.touched, d.sort_a, d.sort_b, d.sort_flags, ex_vcount, ex_voff,
+                    ex_chunks, ex_flag_count, stage_posnor, stage_rgb};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+}
+
+hipEvent_t OsnTsdfHipVolume::take_event() {
+    if (!ev_pool.empty()) {
+        hipEvent_t e = ev_pool.back();
+        ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void OsnTsdfHipVolume::stage_begin(int stage, hipEvent_t& a) {
+    (void)stage;
+    a = nullptr;
+    if (!timing) return;
+    a = take_event();
+    if (a) hipEventRecord(a, stream);
+}
+
+void OsnTsdfHipVolume::stage_end(int stage, hipEvent_t a) {
+    if (!timing || !a) return;
+    hipEvent_t b = take_event();
+    if (!b) return;
+    hipEventRecord(b, stream);
+    ev_pending.push_back({stage, {a, b}});
+}
+
+bool OsnTsdfHipVolume::grow_extract(int32_t blocks) {
+    if (blocks <= ex_blocks_cap && ex_vcount) return true;
+    int32_t want = blocks > 1024 ? blocks : 1024;
+    if (ex_vcount) { hipFree(ex_vcount); hipFree(ex_voff); hipFree(ex_chunks); }
+    ex_vcount = ex_voff = ex_chunks = nullptr;
+    HIP_OK(hipMalloc(&ex_vcount, (size_t)want * sizeof(int32_t)));
+    HIP_OK(hipMalloc(&ex_voff, (size_t)want * sizeof(int32_t)));
+    HIP_OK(hipMalloc(&ex_chunks, ((size_t)want / 1024 + 2) * sizeof(int32_t)));
+    if (!ex_flag_count) {
+        HIP_OK(hipMalloc(&ex_flag_count, 2 * sizeof(int32_t)));
+        HIP_OK(hipMemsetAsync(ex_flag_count, 0, 2 * sizeof(int32_t), stream));
+    }
+    ex_blocks_cap = want;
+    return true;
+}
+
+bool OsnTsdfHipVolume::grow_stage(int64_t verts, bool rgb) {
+    if (verts <= stage_cap && (!rgb || stage_rgb)) return true;
+    const int64_t want = verts > stage_cap ? verts : stage_cap;
+    if (stage_posnor) hipFree(stage_posnor);
+    if (stage_rgb) hipFree(stage_rgb);
+    stage_posnor = nullptr;
+    stage_rgb = nullptr;
+    HIP_OK(hipMalloc(&stage_posnor, (size_t)want * 6 * sizeof(float)));
+    if (rgb) HIP_OK(hipMalloc(&stage_rgb, (size_t)want * 3));
+    stage_cap = want;
+    return true;
+}
+
+namespace {
+
+OsnTsdfHipVolume* make_volume(const OsnTsdfHipConfig& c, bool reference_shape) {
+    if (!(c.voxel_size_m > 0.0f) || c.pool_capacity_blocks <= 0) return nullptr;
+    if (c.storage < kStoreSums || c.storage > kStoreF64) return nullptr;
+    uint32_t slots = c.table_slots;
+    if (slots == 0) slots = next_pow2((uint32_t)c.pool_capacity_blocks * 2u);
+    if (c.hash_kind == kHashTeschnerPow2 && (slots & (slots - 1))) return nullptr;
+    if (c.hash_kind != kHashTeschnerPow2 && c.hash_kind != kHashMixRange) return nullptr;
+    if (slots < 2) return nullptr;
+    if (c.lds_set > 2048 || (c.lds_set > 0 && (c.lds_set & (c.lds_set - 1)))) return nullptr;
+
+    auto* v = new (std::nothrow) OsnTsdfHipVolume();
+    if (!v) return nullptr;
+    v->cfg = c;
+    v->cfg.table_slots = slots;
+    v->reference_shape = reference_shape;
+    const float trunc = c.trunc_m > 0.0f ? c.trunc_m : 4.0f * c.voxel_size_m;
+    VolDev& d = v->d;
+    d.capacity = slots;
+    d.mask = slots - 1;
+    d.hash_kind = c.hash_kind;
+    d.probe_limit = c.probe_limit ? (c.probe_limit < slots ? c.probe_limit : slots) : slots;
+    d.pool_capacity = c.pool_capacity_blocks;
+    d.storage = c.storage;
+    d.voxel = c.voxel_size_m;
+    d.inv_voxel = 1.0f / c.voxel_size_m;     // volume.cpp:148
+    d.trunc = trunc;
+    d.weight_cap = c.weight_cap;
+    const int32_t cs = (int32_t)std::ceil(trunc * d.inv_voxel);  // volume.cpp:171
+    d.steps = cs > 1 ? cs : 1;
+    d.w_max = c.weight_cap > 0.0f ? (int32_t)(c.weight_cap < 65535.0f ? c.weight_cap : 65535.0f)
+                                  : 65535;
+    if (d.w_max < 1) d.w_max = 1;
+    v->lds_set = c.lds_set < 0 ? 1024 : c.lds_set;
+    v->update_threads = c.update_threads > 0 ? c.update_threads : 256;
+    v->update_grid = c.update_grid;
+    if (!v->alloc()) {
+        delete v;
+        return nullptr;
+    }
+    return v;
+}
+
+PointSrc points_src(const float* pos, int32_t n, float cx, float cy, float cz, float radius) {
+    PointSrc s{};
+    s.pos = pos;
+    s.n = n;
+    s.cam[0] = cx; s.cam[1] = cy; s.cam[2] = cz;
+    s.radius_sq = radius > 0.0f ? radius * radius : 0.0f;
+    return s;
+}
+
+PointSrc depth_src(const float* depth, const OsnTsdfHipFrame* f) {
+    PointSrc s{};
+    s.depth = depth;
+    s.W = f->width; s.H = f->height;
+    s.fx = f->fx; s.fy = f->fy; s.cx = f->cx; s.cy = f->cy;
+    std::memcpy(s.R, f->R_wc, sizeof(s.R));
+    s.cam[0] = f->t_wc[0]; s.cam[1] = f->t_wc[1]; s.cam[2] = f->t_wc[2];
+    s.max_range = f->max_range_m;
+    s.radius_sq = f->radius_m > 0.0f ? f->radius_m * f->radius_m : 0.0f;
+    return s;
+}
+
+ProjCam proj_cam(const float* depth, const OsnTsdfHipFrame* f) {
+    ProjCam k{};
+    // R_cw = R_wc^T
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            k.Rcw[i * 3 + j] = f->R_wc[j * 3 + i];
+            k.Rcw64[i * 3 + j] = f->R_wc64[j * 3 + i];
+        }
+    for (int i = 0; i < 3; ++i) { k.c[i] = f->t_wc[i]; k.c64[i] = f->t_wc64[i]; }
+    k.fx = f->fx; k.fy = f->fy; k.cx = f->cx; k.cy = f->cy;
+    k.W = f->width; k.H = f->height;
+    k.depth = depth;
+    return k;
+}
+
+int32_t run_allocate(OsnTsdfHipVolume* v, const PointSrc& src, int mode, bool touched,
+                     const int32_t* coords = nullptr) {
+    hipEvent_t e0, e1;
+    v->stage_begin(0, e0);
+    hipError_t err = launch_begin_call(v->d, v->stream);
+    AllocOpts o;
+    o.mode = mode;
+    o.track_touched = touched;
+    o.lds_set = v->lds_set;
+    if (err == hipSuccess) err = launch_alloc(v->d, src, o, coords, v->stream);
+    v->stage_end(0, e0);
+    if (err != hipSuccess) { v->fail("allocate launch", err); return 2; }
+    if (mode == kAllocNoCas) return 0;
+    v->stage_begin(1, e1);
+    err = launch_assign(v->d, v->stream);
+    v->stage_end(1, e1);
+    if (err != hipSuccess) { v->fail("assign launch", err); return 2; }
+    return 0;
+}
+
+bool valid_frame(const float* d_depth, const OsnTsdfHipFrame* f) {
+    return d_depth && f && f->width > 0 && f->height > 0 && f->fx != 0.0f && f->fy != 0.0f;
+}
+
+template <typename T>
+int32_t read_counter(const OsnTsdfHipVolume* v, size_t offset, T* out) {
+    if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    if (hipMemcpy(out, reinterpret_cast<const char*>(v->d.ctr) + offset, sizeof(T),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return 2;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+OsnTsdfHipVolume* osn_tsdf_hip_create(float voxel_size_m, int32_t pool_capacity_blocks,
+                                      float trunc_m, float weight_cap) {
+    OsnTsdfHipConfig c{};
+    c.voxel_size_m = voxel_size_m;
+    c.pool_capacity_blocks = pool_capacity_blocks;
+    c.trunc_m = trunc_m;
+    c.weight_cap = weight_cap;
+    c.storage = kStoreSums;
+    c.hash_kind = kHashTeschnerPow2;
+    c.lds_set = -1;
+    return make_volume(c, true);
+}
+
+OsnTsdfHipVolume* osn_tsdf_hip_create_ex(const OsnTsdfHipConfig* cfg) {
+    if (!cfg) return nullptr;
+    return make_volume(*cfg, false);
+}
+
+void osn_tsdf_hip_destroy(OsnTsdfHipVolume* v) { delete v; }
+
+int32_t osn_tsdf_hip_valid(const OsnTsdfHipVolume* v) { return v && v->ok ? 1 : 0; }
+
+int32_t osn_tsdf_hip_device_view(const OsnTsdfHipVolume* v, OsnTsdfDeviceView* out) {
+    if (!v || !out || !v->ok) return 1;
+    if (v->d.storage != kStoreSums || v->d.hash_kind != kHashTeschnerPow2) return 3;
+    out->table = v->d.table;
+    out->block_count = &v->d.ctr->block_count;
+    out->drop_count = &v->d.ctr->drop_count;
+    out->block_coord = v->d.block_coord;
+    out->tsdf = v->d.tsdf;
+    out->weight = v->d.weight;
+    out->r = v->d.r;
+    out->g = v->d.g;
+    out->b = v->d.b;
+    out->hash_mask = v->d.mask;
+    out->scratch_base = v->d.capacity;
+    out->pool_capacity = v->d.pool_capacity;
+    out->block_dim = kBlockDim;
+    out->voxel_size_m = v->d.voxel;
+    out->trunc_m = v->d.trunc;
+    out->weight_cap = v->d.weight_cap;
+    return 0;
+}
+
+int32_t osn_tsdf_hip_allocate_blocks(OsnTsdfHipVolume* v, const float* d_positions,
+                                     int32_t n_points, float cam_x, float cam_y, float cam_z,
+                                     float radius_m) {
+    if (!v || !d_positions || n_points <= 0) return 1;
+    if (!v->ok) return 2;
+    return run_allocate(v, points_src(d_positions, n_points, cam_x, cam_y, cam_z, radius_m),
+                        kAllocNormal, false);
+}
+
+int32_t osn_tsdf_hip_allocate_slot_index(OsnTsdfHipVolume* v, const float* d_positions,
+                                         int32_t n_points, float cam_x, float cam_y,
+                                         float cam_z, float radius_m) {
+    return osn_tsdf_hip_allocate_blocks(v, d_positions, n_points, cam_x, cam_y, cam_z, radius_m);
+}
+
+int32_t osn_tsdf_hip_allocate_counting_cas(OsnTsdfHipVolume* v, const float* d_positions,
+                                           int32_t n_points, float cam_x, float cam_y,
+                                           float cam_z, float radius_m) {
+    if (!v || !d_positions || n_points <= 0) return 1;
+    if (!v->ok) return 2;
+    return run_allocate(v, points_src(d_positions, n_points, cam_x, cam_y, cam_z, radius_m),
+                        kAllocCountCas, false);
+}
+
+int32_t osn_tsdf_hip_allocate_no_cas(OsnTsdfHipVolume* v, const float* d_positions,
+                                     int32_t n_points, float cam_x, float cam_y, float cam_z,
+                                     float radius_m) {
+    if (!v || !d_positions || n_points <= 0) return 1;
+    if (!v->ok) return 2;
+    return run_allocate(v, points_src(d_positions, n_points, cam_x, cam_y, cam_z, radius_m),
+                        kAllocNoCas, false);
+}
+
+int32_t osn_tsdf_hip_allocate_probe_depth(OsnTsdfHipVolume* v, const float* d_positions,
+                                          int32_t n_points, float cam_x, float cam_y,
+                                          float cam_z, float radius_m, uint64_t* lane_steps,
+                                          uint64_t* wave_steps, uint64_t* hist) {
+    if (lane_steps) *lane_steps = 0;
+    if (wave_steps) *wave_steps = 0;
+    if (!v || !d_positions || n_points <= 0) return 1;
+    if (!v->ok) return 2;
+    const int32_t rc = run_allocate(
+        v, points_src(d_positions, n_points, cam_x, cam_y, cam_z, radius_m), kAllocProbe, false);
+    if (rc) return rc;
+    return osn_tsdf_hip_probe_stats(v, hist, lane_steps, wave_steps, nullptr);
+}
+
+int32_t osn_tsdf_hip_update_voxels(OsnTsdfHipVolume* v, const float* d_positions,
+                                   int32_t n_points, float cam_x, float cam_y, float cam_z,
+                                   float radius_m) {
+    if (!v || !d_positions || n_points <= 0) return 1;
+    if (!v->ok) return 2;
+    if (v->d.storage != kStoreSums) return 3;
+    hipEvent_t e;
+    v->stage_begin(2, e);
+    const hipError_t err = launch_scatter_update(
+        v->d, points_src(d_positions, n_points, cam_x, cam_y, cam_z, radius_m), v->stream);
+    v->stage_end(2, e);
+    if (err != hipSuccess) { v->fail("update launch", err); return 2; }
+    return 0;
+}
+
+int32_t osn_tsdf_hip_allocate_depth(OsnTsdfHipVolume* v, const float* d_depth,
+                                    const OsnTsdfHipFrame* frame) {
+    if (!v || !valid_frame(d_depth, frame)) return 1;
+    if (!v->ok) return 2;
+    return run_allocate(v, depth_src(d_depth, frame), kAllocNormal, true);
+}
+
+int32_t osn_tsdf_hip_update_depth(OsnTsdfHipVolume* v, const float* d_depth,
+                                  const OsnTsdfHipFrame* frame) {
+    if (!v || !valid_frame(d_depth, frame)) return 1;
+    if (!v->ok) return 2;
+    if (v->d.storage == kStoreSums) return 3;
+    hipEvent_t e;
+    v->stage_begin(2, e);
+    const hipError_t err = launch_proj_update(v->d, proj_cam(d_depth, frame), v->update_grid,
+                                              v->update_threads, v->stream);
+    v->stage_end(2, e);
+    if (err != hipSuccess) { v->fail("projective update launch", err); return 2; }
+    return 0;
+}
+
+int32_t osn_tsdf_hip_integrate_depth(OsnTsdfHipVolume* v, const float* d_depth,
+                                     const OsnTsdfHipFrame* frame) {
+    const int32_t rc = osn_tsdf_hip_allocate_depth(v, d_depth, frame);
+    if (rc) return rc;
+    return osn_tsdf_hip_update_depth(v, d_depth, frame);
+}
+
+int32_t osn_tsdf_hip_backproject(const float* d_depth, const OsnTsdfHipFrame* frame,
+                                 float* d_points_out) {
+    if (!valid_frame(d_depth, frame) || !d_points_out) return 1;
+    const hipError_t err = launch_backproject(depth_src(d_depth, frame), d_points_out, nullptr);
+    if (err != hipSuccess) return 2;
+    return hipDeviceSynchronize() == hipSuccess ? 0 : 2;
+}
+
+int32_t osn_tsdf_hip_insert_blocks(OsnTsdfHipVolume* v, const int32_t* d_coords, int32_t n) {
+    if (!v || !d_coords || n <= 0) return 1;
+    if (!v->ok) return 2;
+    PointSrc s{};
+    s.n = n;
+    return run_allocate(v, s, kAllocNormal, false, d_coords);
+}
+
+int32_t osn_tsdf_hip_extract_mesh_device(OsnTsdfHipVolume* v, float* d_posnor, uint8_t* d_rgb,
+                                         int32_t vert_cap, float min_weight, float iso,
+                                         int32_t* d_count) {
+    if (!v || !d_posnor || vert_cap <= 0 || !d_count) return 1;
+    if (!v->ok) return 2;
+    if (!v->grow_extract(v->d.pool_capacity)) { v->ok = false; return 2; }
+    ExtractArgs a{};
+    a.min_weight = min_weight;
+    a.iso = iso;
+    a.n_blocks = -1;  // read from the device: no host synchronisation
+    a.block_vcount = v->ex_vcount;
+    a.block_voff = v->ex_voff;
+    a.chunk_sums = v->ex_chunks;
+    a.scan_flag = v->ex_flag_count;
+    a.posnor = d_posnor;
+    a.rgb = d_rgb;
+    a.cap = vert_cap;
+    a.count_out = d_count;
+    hipEvent_t e;
+    v->stage_begin(3, e);
+    hipError_t err = launch_extract(v->d, a, false, v->stream);
+    if (err == hipSuccess) err = launch_extract(v->d, a, true, v->stream);
+    v->stage_end(3, e);
+    if (err != hipSuccess) { v->fail("extract launch", err); return 2; }
+    return 0;
+}
+
+int32_t osn_tsdf_hip_extract_mesh(OsnTsdfHipVolume* v, float* buffer_posnor, uint8_t* buffer_rgb,
+                                  int32_t vert_cap, float min_weight, float iso) {
+    if (!v || !buffer_posnor || vert_cap <= 0) return -1;
+    if (!v->ok) return -1;
+    const int32_t n_blocks = osn_tsdf_hip_block_count(v);  // synchronises
+    if (n_blocks <= 0) return 0;
+    if (!v->grow_extract(n_blocks)) { v->ok = false; return -1; }
+    ExtractArgs a{};
+    a.min_weight = min_weight;
+    a.iso = iso;
+    a.n_blocks = n_blocks;
+    a.block_vcount = v->ex_vcount;
+    a.block_voff = v->ex_voff;
+    a.chunk_sums = v->ex_chunks;
+    a.scan_flag = v->ex_flag_count;
+    a.cap = vert_cap;
+    a.count_out = v->ex_flag_count + 1;
+    hipEvent_t e;
+    v->stage_begin(3, e);
+    hipError_t err = launch_extract(v->d, a, false, v->stream);
+    if (err != hipSuccess) { v->fail("extract count launch", err); return -1; }
+    int32_t total = 0;
+    if (hipStreamSynchronize(v->stream) != hipSuccess ||
+        hipMemcpy(&total, a.count_out, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+        v->ok = false;
+        return -1;
+    }
+    if (total <= 0) { v->stage_end(3, e); return total; }  // 0, or -1 on overflow
+    if (!v->grow_stage(total, buffer_rgb != nullptr)) { v->ok = false; return -1; }
+    a.posnor = v->stage_posnor;
+    a.rgb = buffer_rgb ? v->stage_rgb : nullptr;
+    err = launch_extract(v->d, a, true, v->stream);
+    v->stage_end(3, e);
+    if (err != hipSuccess) { v->fail("extract emit launch", err); return -1; }
+    if (hipMemcpyAsync(buffer_posnor, v->stage_posnor, (size_t)total * 6 * sizeof(float),
+                       hipMemcpyDeviceToHost, v->stream) != hipSuccess)
+        return -1;
+    if (buffer_rgb && hipMemcpyAsync(buffer_rgb, v->stage_rgb, (size_t)total * 3,
+                                     hipMemcpyDeviceToHost, v->stream) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(v->stream) != hipSuccess) return -1;
+    return total;
+}
+
+int32_t osn_tsdf_hip_reset(OsnTsdfHipVolume* v) {
+    if (!v) return 1;
+    return v->reset() ? 0 : 2;
+}
+
+int32_t osn_tsdf_hip_block_count(const OsnTsdfHipVolume* v) {
+    if (!v || !v->ok) return 0;
+    int32_t n = 0;
+    if (read_counter(v, offsetof(Counters, block_count), &n)) return 0;
+    return n;
+}
+
+uint64_t osn_tsdf_hip_drop_count(const OsnTsdfHipVolume* v) {
+    if (!v || !v->ok) return 0;
+    unsigned long long n = 0;
+    if (read_counter(v, offsetof(Counters, drop_count), &n)) return 0;
+    return n;
+}
+
+int32_t osn_tsdf_hip_touched_count(const OsnTsdfHipVolume* v) {
+    if (!v || !v->ok) return 0;
+    int32_t n = 0;
+    if (read_counter(v, offsetof(Counters, touched_count), &n)) return 0;
+    return n;
+}
+
+void osn_tsdf_hip_synchronize(const OsnTsdfHipVolume* v) {
+    if (v && v->stream) hipStreamSynchronize(v->stream);
+}
+
+int32_t osn_tsdf_hip_block_keys(const OsnTsdfHipVolume* v, int64_t* host_keys, int32_t cap) {
+    if (!v || !host_keys || cap < 0 || !v->ok) return -1;
+    const int32_t n_all = osn_tsdf_hip_block_count(v);
+    const int32_t n = n_all < cap ? n_all : cap;
+    if (n <= 0) return 0;
+    std::vector<int32_t> c((size_t)n * 3);
+    if (hipMemcpy(c.data(), v->d.block_coord, c.size() * sizeof(int32_t),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    for (int32_t i = 0; i < n; ++i) host_keys[i] = pack_coord(c[i * 3], c[i * 3 + 1], c[i * 3 + 2]);
+    return n;
+}
+
+int32_t osn_tsdf_hip_read_voxels(const OsnTsdfHipVolume* v, int32_t first, int32_t n,
+                                 float* host_tsdf, float* host_weight) {
+    if (!v || !v->ok || first < 0 || n <= 0 || first + n > v->d.pool_capacity) return 1;
+    if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    const size_t o = (size_t)first * kBlockVoxels;
+    const size_t cnt = (size_t)n * kBlockVoxels;
+    if (v->d.storage == kStoreSums) {
+        if (host_tsdf && hipMemcpy(host_tsdf, v->d.tsdf + o, cnt * 4, hipMemcpyDeviceToHost)) return 2;
+        if (host_weight && hipMemcpy(host_weight, v->d.weight + o, cnt * 4, hipMemcpyDeviceToHost)) return 2;
+        return 0;
+    }
+    if (v->d.storage == kStoreF32) {
+        std::vector<float> raw(cnt * 2);
+        if (hipMemcpy(raw.data(), static_cast<float*>(v->d.vox) + o * 2, cnt * 8,
+                      hipMemcpyDeviceToHost))
+            return 2;
+        for (size_t i = 0; i < cnt; ++i) {
+            if (host_tsdf) host_tsdf[i] = raw[i * 2];
+            if (host_weight) host_weight[i] = raw[i * 2 + 1];
+        }
+        return 0;
+    }
+    if (v->d.storage == kStoreF16) {
+        std::vector<uint32_t> raw(cnt);
+        if (hipMemcpy(raw.data(), static_cast<uint32_t*>(v->d.vox) + o, cnt * 4,
+                      hipMemcpyDeviceToHost))
+            return 2;
+        for (size_t i = 0; i < cnt; ++i) {
+            // IEEE binary16 -> float, exact
+            const uint32_t h = raw[i] & 0xffffu;
+            const uint32_t s = (h >> 15) & 1u, e = (h >> 10) & 31u, m = h & 1023u;
+            float f;
+            if (e == 0) f = std::ldexp((float)m, -24);
+            else if (e == 31) f = m ? NAN : INFINITY;
+            else f = std::ldexp((float)(m | 1024u), (int)e - 25);
+            if (host_tsdf) host_tsdf[i] = s ? -f : f;
+            if (host_weight) host_weight[i] = (float)(raw[i] >> 16);
+        }
+        return 0;
+    }
+    std::vector<double> raw(cnt * 2);
+    if (hipMemcpy(raw.data(), static_cast<double*>(v->d.vox) + o * 2, cnt * 16,
+                  hipMemcpyDeviceToHost))
+        return 2;
+    for (size_t i = 0; i < cnt; ++i) {
+        if (host_tsdf) host_tsdf[i] = (float)raw[i * 2];
+        if (host_weight) host_weight[i] = (float)raw[i * 2 + 1];
+    }
+    return 0;
+}
+
+int32_t osn_tsdf_hip_read_voxels_f64(const OsnTsdfHipVolume* v, int32_t first, int32_t n,
+                                     double* host_tsdf, double* host_weight) {
+    if (!v || !v->ok || first < 0 || n <= 0 || first + n > v->d.pool_capacity) return 1;
+    if (v->d.storage != kStoreF64) return 3;
+    if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    const size_t cnt = (size_t)n * kBlockVoxels;
+    std::vector<double> raw(cnt * 2);
+    if (hipMemcpy(raw.data(), static_cast<double*>(v->d.vox) + (size_t)first * kBlockVoxels * 2,
+                  cnt * 16, hipMemcpyDeviceToHost))
+        return 2;
+    for (size_t i = 0; i < cnt; ++i) {
+        if (host_tsdf) host_tsdf[i] = raw[i * 2];
+        if (host_weight) host_weight[i] = raw[i * 2 + 1];
+    }
+    return 0;
+}
+
+int32_t osn_tsdf_hip_probe_stats(const OsnTsdfHipVolume* v, uint64_t* hist64,
+                                 uint64_t* lane_steps, uint64_t* wave_steps, uint64_t* lookups) {
+    if (!v || !v->ok) return 1;
+    Counters c;
+    if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    if (hipMemcpy(&c, v->d.ctr, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) return 2;
+    if (hist64) for (int i = 0; i < kProbeHistBins; ++i) hist64[i] = c.probe_hist[i];
+    if (lane_steps) *lane_steps = c.lane_steps;
+    if (wave_steps) *wave_steps = c.wave_steps;
+    if (lookups) *lookups = c.lookups;
+    return 0;
+}
+
+int32_t osn_tsdf_hip_set_timing(OsnTsdfHipVolume* v, int32_t enable) {
+    if (!v) return 1;
+    v->timing = enable != 0;
+    return 0;
+}
+
+int32_t osn_tsdf_hip_read_timing(OsnTsdfHipVolume* v, double* ms4, int64_t* calls4) {
+    if (!v) return 1;
+    if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    for (auto& p : v->ev_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            v->ms[p.first] += ms;
+            v->calls[p.first] += 1;
+        }
+        v->ev_pool.push_back(p.second.first);
+        v->ev_pool.push_back(p.second.second);
+    }
+    v->ev_pending.clear();
+    for (int i = 0; i < kStages; ++i) {
+        if (ms4) ms4[i] = v->ms[i];
+        if (calls4) calls4[i] = v->calls[i];
+        v->ms[i] = 0.0;
+        v->calls[i] = 0;
+    }
+    return 0;
+}
+
+void* osn_tsdf_hip_stream(const OsnTsdfHipVolume* v) { return v ? (void*)v->stream : nullptr; }
+
+}  // extern "C"
