@@ -1,0 +1,324 @@
+"""Benchmark: frames/s of depth-frame TSDF integration at 640x480, 1 cm voxels.
+
+Workload ("precise" scene): a seeded 6x6x3 m room with 20 spheres and 10 boxes,
+640x480 planar depth frames rendered on the GPU with N(0, 0.0012 z^2) noise and
+1% dropouts, fx=fy=320, cx=320, cy=240; voxel 1 cm, truncation 4 cm, table
+2^22 slots, pool 2^21 blocks, weight cap 32. A step is one frame: allocate
+(FP32 back-projection + band walk + insert), index assignment, projective
+update. All depth frames are resident in HBM before the timed region.
+
+Reported:
+  value          frames/s of the FP16-storage pipeline (FP32 arithmetic, FP16
+                 normalised TSDF + u16 weight), warmup frames untimed;
+  fp32_*         the same frames through plain FP32 storage, for the ratio;
+  precision      max / RMS |dtsdf| against the FP64 oracle kernel on the same
+                 frames, in truncation units and in voxels;
+  roofline       the update kernel: algorithmic bytes (touched blocks x 512
+                 voxels x (read + write) x bytes/voxel + depth image + block
+                 list) / its average duration from HIP events on the volume's
+                 stream, against 8 TB/s;
+  cpu_baseline   the reference CPU arm (compiled from the reference sources,
+                 oracle/_ref) integrating the same frames' back-projected points
+                 with all host threads, on a bounded sample of frames.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "what-irregularity-costs_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def render_frames(H, S, cfg, seed, n):
+    import torch
+    scene, poses = S.build(dict(cfg, frames=n), seed)
+    W, Hh = cfg["width"], cfg["height"]
+    fx, fy, cx, cy = S.intrinsics(W, Hh)
+    sc = scene.to_c()
+    depth = torch.empty((n, Hh * W), dtype=torch.float32, device="cuda")
+    frames = []
+    for k, (R, t) in enumerate(poses):
+        fr = H.make_frame(W, Hh, fx, fy, cx, cy, R, t)
+        if S.render_depth(sc, fr, cfg["noise_k"], cfg["dropout"], seed, k, depth[k]) != 0:
+            raise RuntimeError("render failed")
+        frames.append(fr)
+    torch.cuda.synchronize()
+    return frames, depth
+
+
+def make_volume(H, cfg, storage):
+    return H.HipVolume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 32.0, storage=storage)
+
+
+def run_plain(H, cfg, storage, scenes, warmup, steps):
+    """One scene at a time, one frame per call: 3 launches per frame."""
+    import torch
+    total = 0.0
+    blocks = 0
+    for frames, depth in scenes:
+        v = make_volume(H, cfg, storage)
+        for k in range(warmup):
+            if v.integrate_depth(depth[k], frames[k]):
+                raise RuntimeError("integrate failed")
+        v.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(warmup, warmup + steps):
+            if v.integrate_depth(depth[k], frames[k]):
+                raise RuntimeError("integrate failed")
+        v.synchronize()
+        total += time.perf_counter() - t0
+        blocks += v.block_count
+        v.close()
+    return total, blocks
+
+
+def make_batch(H, cfg, storage, scenes):
+    vols = [make_volume(H, cfg, storage) for _ in scenes]
+    b = H.Batch(vols)
+    n = scenes[0][1].shape[0]
+    depths = [[sc[1][k] for sc in scenes] for k in range(n)]
+    frames = [[sc[0][k] for sc in scenes] for k in range(n)]
+    if b.set_schedule(depths, frames):
+        raise RuntimeError("set_schedule failed")
+    return b, vols
+
+
+def run_batched(H, cfg, storage, scenes, warmup, steps, graph=True):
+    """All scenes per step in one launch per stage, replayed from a hipGraph."""
+    import torch
+    b, vols = make_batch(H, cfg, storage, scenes)
+    if b.run(warmup, use_graph=graph):
+        raise RuntimeError("batch run failed")
+    b.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if b.run(steps, use_graph=graph):
+        raise RuntimeError("batch run failed")
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    blocks = sum(v.block_count for v in vols)
+    b.close()
+    for v in vols:
+        v.close()
+    return dt, blocks
+
+
+def run_batched_instrumented(H, cfg, storage, scenes, warmup, steps, bytes_per_voxel):
+    """Eager batched steps with HIP events per stage and per-step touched counts."""
+    b, vols = make_batch(H, cfg, storage, scenes)
+    b.run(warmup)
+    b.synchronize()
+    b.read_timing()
+    b.set_timing(True)
+    touched = []
+    for _ in range(steps):
+        b.run(1)
+        touched.append(sum(v.touched_count for v in vols))  # synchronises
+    ms, calls = b.read_timing()
+    W, Hh = cfg["width"], cfg["height"]
+    tb = np.array(touched, np.float64)
+    alg_bytes = tb * 512 * 2 * bytes_per_voxel + len(scenes) * W * Hh * 4 + tb * 12
+    b.close()
+    for v in vols:
+        v.close()
+    return ms, calls, tb, alg_bytes
+
+
+def precision(H, cfg, frames, depth, n):
+    """FP16 and FP32 states against the FP64 oracle kernel after n frames."""
+    vols = {}
+    for name, st in (("f64", H.STORE_F64), ("f32", H.STORE_F32), ("f16", H.STORE_F16)):
+        v = make_volume(H, cfg, st)
+        for k in range(n):
+            v.integrate_depth(depth[k], frames[k])
+        v.synchronize()
+        vols[name] = v
+    keys = vols["f64"].keys()
+    t64, w64 = vols["f64"].read_voxels_f64()
+    vols["f64"].close()
+    out = {}
+    for name in ("f32", "f16"):
+        k, (t, w) = vols[name].keys(), vols[name].read_voxels()
+        vols[name].close()
+        assert np.array_equal(k, keys), "key sets differ between storages"
+        obs = w64 > 0
+        assert np.array_equal(w.astype(np.float64)[obs], w64[obs])
+        err = np.abs(t.astype(np.float64) - t64)[obs]
+        vox = cfg["trunc"] / cfg["voxel"]
+        out[name] = {"max_trunc": float(err.max()), "rms_trunc": float(np.sqrt((err ** 2).mean())),
+                     "max_voxels": float(err.max() * vox),
+                     "rms_voxels": float(np.sqrt((err ** 2).mean()) * vox),
+                     "observed_voxels": int(obs.sum())}
+    return out
+
+
+def cpu_baseline(H, cfg, frames, depth, budget_s=12.0, max_frames=400):
+    """Reference CPU arm (all threads) on the same frames' points."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import bindings as B
+    kind = "reference" if B.ref_available() else "port"
+    W, Hh = cfg["width"], cfg["height"]
+    buf = torch.empty(W * Hh * 3, dtype=torch.float32, device="cuda")
+    if kind == "reference":
+        vol = B.RefVolume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 32.0)
+        threads = vol.threads
+    else:
+        vol = B.Volume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 32.0)
+        threads = 1
+    spent = 0.0
+    n = 0
+    times = []
+    t_start = time.perf_counter()
+    for k in range(min(max_frames, len(frames))):
+        H.backproject(depth[k], frames[k], buf)
+        p = buf.cpu().numpy().reshape(-1, 3)
+        pts = np.ascontiguousarray(p[~np.isnan(p[:, 0])].ravel())
+        cam = tuple(frames[k].t_wc[:])
+        t0 = time.perf_counter()
+        vol.allocate(pts, cam)
+        vol.update(pts, cam)
+        dt = time.perf_counter() - t0
+        if k > 0:  # the first frame is warmup
+            times.append(dt)
+            spent += dt
+            n += 1
+        if spent > budget_s or time.perf_counter() - t_start > 3 * budget_s:
+            break
+    t = np.sort(np.array(times))
+    pct = lambda q: float(t[min(len(t) - 1, int(len(t) * q))])  # nearest rank
+    return {"value": n / spent, "unit": "frames/s", "cores": int(threads), "kind": kind,
+            "sample": f"frames 1..{n} of the same trajectory (frame 0 warmup), allocate+update "
+                      f"per frame on back-projected points, p50 {pct(0.5) * 1e3:.1f} ms, "
+                      f"p95 {pct(0.95) * 1e3:.1f} ms"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--scenes", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the batched fp16 run and one 256 MiB device copy (the copy "
+                         "calibrates HBM byte counters), for rocprofv3 passes")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        sys.exit(2)
+
+    import torch
+    import osn_tsdf_hip as H
+    import scenes as S
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    H.lib()
+    cfg = dict(S.PRECISE)
+    n = args.warmup + args.steps
+    log(f"rendering {args.scenes} scenes x {n} frames {cfg['width']}x{cfg['height']}")
+    scenes = [render_frames(H, S, cfg, args.seed + i, n) for i in range(args.scenes)]
+    frames_timed = args.steps * args.scenes
+
+    if args.profile_only:
+        src = torch.empty(1 << 26, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        dst.copy_(src)  # 256 MiB read + 256 MiB written
+        torch.cuda.synchronize()
+        dt, _ = run_batched(H, cfg, H.STORE_F16, scenes, args.warmup, args.steps,
+                            graph=not args.no_graph)
+        log(f"batched fp16 {frames_timed / dt:.0f} frames/s")
+        return
+
+    log("batched fp16 (timed)")
+    dt_b16, blocks = run_batched(H, cfg, H.STORE_F16, scenes, args.warmup, args.steps,
+                                 graph=not args.no_graph)
+    log("plain fp32 (timed)")
+    dt_p32, _ = run_plain(H, cfg, H.STORE_F32, scenes, args.warmup, args.steps)
+    log("batched fp32 / plain fp16 (timed)")
+    dt_b32, _ = run_batched(H, cfg, H.STORE_F32, scenes, args.warmup, args.steps,
+                            graph=not args.no_graph)
+    dt_p16, _ = run_plain(H, cfg, H.STORE_F16, scenes, args.warmup, args.steps)
+    fps = {k: frames_timed / v for k, v in
+           (("batched_fp16", dt_b16), ("plain_fp32", dt_p32), ("batched_fp32", dt_b32),
+            ("plain_fp16", dt_p16))}
+    log(" ".join(f"{k} {v:.0f}" for k, v in fps.items()))
+
+    log("instrumented batched runs")
+    ms16, calls16, tb16, bytes16 = run_batched_instrumented(H, cfg, H.STORE_F16, scenes,
+                                                            args.warmup, args.steps, 4)
+    ms32, calls32, tb32, bytes32 = run_batched_instrumented(H, cfg, H.STORE_F32, scenes,
+                                                            args.warmup, args.steps, 8)
+    upd_ms16 = ms16[2] / max(calls16[2], 1)
+    achieved16 = bytes16.sum() / (ms16[2] * 1e-3) / 1e9
+    achieved32 = bytes32.sum() / (ms32[2] * 1e-3) / 1e9
+
+    prec = None
+    if not args.no_precision:
+        log("precision against the FP64 oracle kernel")
+        prec = precision(H, cfg, scenes[0][0], scenes[0][1], n)
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(H, cfg, scenes[0][0], scenes[0][1])
+
+    out = {
+        "metric": "frames/s 640x480@1cm: batched fp16-TSDF vs plain fp32, max err <=0.01 trunc",
+        "value": round(fps["batched_fp16"], 1),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt_b16 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded analytic scenes, GPU-rendered depth with noise and dropouts)",
+        "config": {"workload": "precise: 6x6x3 m rooms with 20 spheres + 10 boxes, 640x480 depth, "
+                               "voxel 1 cm, trunc 4 cm, table 2^22, weight cap 32",
+                   "scenes_per_step": args.scenes, "storage": "fp16 tsdf/trunc + u16 weight",
+                   "graph": not args.no_graph, "frames_per_scene": n,
+                   "blocks_allocated": int(blocks)},
+        "plain_fp32_frames_per_s": round(fps["plain_fp32"], 1),
+        "batched_fp16_over_plain_fp32": round(fps["batched_fp16"] / fps["plain_fp32"], 3),
+        "batched_fp32_frames_per_s": round(fps["batched_fp32"], 1),
+        "plain_fp16_frames_per_s": round(fps["plain_fp16"], 1),
+        "stage_ms_per_step_fp16": {s: round(float(ms16[i] / max(calls16[i], 1)), 5)
+                                   for i, s in enumerate(H.STAGES[:3])},
+        "stage_ms_per_step_fp32": {s: round(float(ms32[i] / max(calls32[i], 1)), 5)
+                                   for i, s in enumerate(H.STAGES[:3])},
+        "touched_blocks_per_step": round(float(tb16.mean()), 1),
+        "update_fp32_gbs": round(float(achieved32), 1),
+        "precision_vs_fp64": prec,
+        "roofline": {"bound": "hbm", "achieved": round(float(achieved16), 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(float(achieved16 / HBM_PEAK_GBS), 4), "traffic": None,
+                     "kernel": "k_proj_update_batch<fp16>",
+                     "avg_launch_ms": round(float(upd_ms16), 5),
+                     "bytes_per_launch": round(float(bytes16.mean()), 0)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

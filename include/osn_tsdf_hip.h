@@ -144,6 +144,7 @@ typedef struct {
     int32_t lds_set;        /* per-workgroup pre-hash entries: power of two <= 2048, 0 off, -1 default */
     int32_t update_threads; /* projective update workgroup size (128/256/512), 0 default */
     int32_t update_grid;    /* projective update workgroups, 0 default */
+    int32_t alloc_grid;     /* allocate workgroups (each loops over pixel tiles), 0 default */
 } OsnTsdfHipConfig;
 
 /* NULL on bad parameters or allocation failure. */
@@ -218,6 +219,36 @@ int32_t osn_tsdf_hip_read_timing(OsnTsdfHipVolume* v, double* ms4, int64_t* call
 /* The volume's stream (a hipStream_t), for callers that order their own work
  * against it. */
 void* osn_tsdf_hip_stream(const OsnTsdfHipVolume* v);
+
+/* ---- batches: many independent scenes per launch ------------------------- */
+
+typedef struct OsnTsdfHipBatch OsnTsdfHipBatch;
+
+/* Group volumes (all projective, all the same storage) to be integrated
+ * together. The batch runs on its own stream; while it exists the volumes
+ * must not be integrated through the single-volume calls. NULL on error. */
+OsnTsdfHipBatch* osn_tsdf_hip_batch_create(OsnTsdfHipVolume* const* volumes, int32_t n_volumes);  /* 1..64 volumes */
+void osn_tsdf_hip_batch_destroy(OsnTsdfHipBatch* b);
+/* Upload the schedule: step k of scene s integrates d_depths[k*n + s] with
+ * frames[k*n + s]; a NULL depth pointer leaves the scene idle in that step.
+ * Images may differ in size between scenes. Resets the step cursor to 0. */
+int32_t osn_tsdf_hip_batch_set_schedule(OsnTsdfHipBatch* b, int32_t n_steps,
+                                        const float* const* d_depths,
+                                        const OsnTsdfHipFrame* frames);
+/* Integrate the next n_steps steps (allocate, assign, projective update for
+ * every active scene: three launches per step). With use_graph the per-step
+ * chain is captured once in a hipGraph and replayed, the step index living
+ * in device memory. Asynchronous; 1 when the schedule would be exceeded. */
+int32_t osn_tsdf_hip_batch_run(OsnTsdfHipBatch* b, int32_t n_steps, int32_t use_graph);
+void osn_tsdf_hip_batch_synchronize(const OsnTsdfHipBatch* b);
+/* Batch launch shape: allocate workgroups, update workgroups and update
+ * workgroup size (0 keeps the default). Invalidates a captured graph. */
+int32_t osn_tsdf_hip_batch_tune(OsnTsdfHipBatch* b, int32_t alloc_grid, int32_t update_grid,
+                                int32_t update_threads, int32_t lds_set);
+/* Stage timing for batched steps, as osn_tsdf_hip_read_timing (stages 0-2;
+ * with use_graph the whole step is reported as stage 0). */
+int32_t osn_tsdf_hip_batch_set_timing(OsnTsdfHipBatch* b, int32_t enable);
+int32_t osn_tsdf_hip_batch_read_timing(OsnTsdfHipBatch* b, double* ms4, int64_t* calls4);
 
 #ifdef __cplusplus
 }

@@ -668,7 +668,8 @@ int32_t orc_backproject_f32(const float* depth, int32_t W, int32_t H, float fx, 
  * The FP32 part is the device's exact expression sequence:
  *   o  = (float)(b * 8) * voxel + 0.5f * voxel - c        (block origin centre)
  *   d0 = R_cw * o                                         (row dot, left to right)
- *   p  = d0 + lx * ex + ly * ey + lz * ez,  e* = voxel * R_cw[:, *]
+ *   row = (d0 + ly * ey) + lz * ez,  e* = voxel * R_cw[:, *]
+ *   p   = row + lx * ex                   (one row per (ly, lz), shared along x)
  * with each product rounded separately (no FMA).
  */
 typedef struct {
@@ -702,7 +703,7 @@ static ProjDecision proj_decide(const OrcProjCam* k, const float* depth, float v
     const float fl[3] = {(float)lx, (float)ly, (float)lz};
     float p[3];
     for (int i = 0; i < 3; ++i)
-        p[i] = d0[i] + fl[0] * e[0][i] + fl[1] * e[1][i] + fl[2] * e[2][i];
+        p[i] = (d0[i] + fl[1] * e[1][i] + fl[2] * e[2][i]) + fl[0] * e[0][i];
     if (!(p[2] > 0.0f)) return r;
     const float iz = 1.0f / p[2];
     const float uf = k->fx * (p[0] * iz) + k->cx + 0.5f;

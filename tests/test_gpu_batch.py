@@ -1,0 +1,155 @@
+"""Batched multi-scene integration against per-scene integration.
+
+Each scene's arithmetic is the same as through the single-volume entry points
+and no two scenes share memory, so the batched state must be byte-identical
+to integrating every scene on its own, whatever the launch shape, and a
+captured-graph replay must match eager launches after every step.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def H():
+    import osn_tsdf_hip
+    assert torch.cuda.is_available()
+    return osn_tsdf_hip
+
+
+@pytest.fixture(scope="module")
+def S():
+    import scenes
+    return scenes
+
+
+SIZES = [(320, 240), (640, 480), (160, 120)]
+
+
+def make_inputs(H, S, n_steps, seed0=40):
+    cfg = S.PRECISE
+    depths, frames = [], []
+    for k in range(n_steps):
+        depths.append([None] * len(SIZES))
+        frames.append([None] * len(SIZES))
+    for s, (W, Hh) in enumerate(SIZES):
+        scene, poses = S.build(dict(cfg, frames=n_steps), seed0 + s)
+        fx, fy, cx, cy = S.intrinsics(W, Hh)
+        sc = scene.to_c()
+        for k, (R, t) in enumerate(poses):
+            if s == 2 and k % 3 == 1:
+                continue  # scene 2 idles every third step
+            fr = H.make_frame(W, Hh, fx, fy, cx, cy, R, t)
+            d = torch.empty(W * Hh, dtype=torch.float32, device="cuda")
+            assert S.render_depth(sc, fr, cfg["noise_k"], cfg["dropout"], seed0 + s, k, d) == 0
+            depths[k][s] = d
+            frames[k][s] = fr
+    torch.cuda.synchronize()
+    return depths, frames
+
+
+def volumes(H, storage, tables=(1 << 18, 1 << 20, 1 << 16)):
+    return [H.HipVolume(0.01, t // 2, 0.04, 32.0, storage=storage) for t in tables]
+
+
+def state(v):
+    k = v.keys()
+    t, w = v.read_voxels()
+    return k, t.view(np.uint32), w.view(np.uint32)
+
+
+def same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("storage", [2, 1])
+def test_batched_equals_per_scene(H, S, storage):
+    n_steps = 9
+    depths, frames = make_inputs(H, S, n_steps)
+    solo = volumes(H, storage)
+    for k in range(n_steps):
+        for s, v in enumerate(solo):
+            if depths[k][s] is not None:
+                assert v.integrate_depth(depths[k][s], frames[k][s]) == 0
+    batched = volumes(H, storage)
+    b = H.Batch(batched)
+    assert b.set_schedule(depths, frames) == 0
+    assert b.run(4) == 0
+    assert b.run(5) == 0
+    assert b.run(1) == 1  # past the schedule
+    b.synchronize()
+    for v1, v2 in zip(solo, batched):
+        assert v1.block_count == v2.block_count > 0
+        assert same(state(v1), state(v2))
+    b.close()
+    # after the batch is gone the volumes continue through the single calls
+    for s, v in enumerate(batched):
+        if depths[0][s] is not None:
+            assert v.integrate_depth(depths[0][s], frames[0][s]) == 0
+            assert solo[s].integrate_depth(depths[0][s], frames[0][s]) == 0
+    for v1, v2 in zip(solo, batched):
+        assert same(state(v1), state(v2))
+
+
+def test_graph_replay_matches_eager_every_step(H, S):
+    n_steps = 6
+    depths, frames = make_inputs(H, S, n_steps, seed0=70)
+    eager_v = volumes(H, 2)
+    graph_v = volumes(H, 2)
+    be = H.Batch(eager_v)
+    bg = H.Batch(graph_v)
+    assert be.set_schedule(depths, frames) == 0
+    assert bg.set_schedule(depths, frames) == 0
+    for k in range(n_steps):
+        assert be.run(1, use_graph=False) == 0
+        assert bg.run(1, use_graph=True) == 0
+        be.synchronize()
+        bg.synchronize()
+        # eager work of the program's own between replays
+        _ = torch.arange(1000, device="cuda").sum().item()
+        for v1, v2 in zip(eager_v, graph_v):
+            assert v1.block_count == v2.block_count
+    for v1, v2 in zip(eager_v, graph_v):
+        assert same(state(v1), state(v2))
+
+
+def test_batch_tuning_keeps_results(H, S):
+    n_steps = 4
+    depths, frames = make_inputs(H, S, n_steps, seed0=90)
+    ref = volumes(H, 2)
+    b0 = H.Batch(ref)
+    b0.set_schedule(depths, frames)
+    b0.run(n_steps)
+    b0.synchronize()
+    for alloc_grid, upd_grid, threads, lds in ((64, 128, 128, 0), (0, 0, 512, 256)):
+        vs = volumes(H, 2)
+        b = H.Batch(vs)
+        assert b.tune(alloc_grid, upd_grid, threads, lds) == 0
+        b.set_schedule(depths, frames)
+        assert b.run(n_steps, use_graph=True) == 0
+        b.synchronize()
+        for v1, v2 in zip(ref, vs):
+            assert same(state(v1), state(v2))
+        b.close()
+
+
+def test_batch_rejects_bad_groups(H):
+    a = H.HipVolume(0.01, 1024, 0.04, 32.0, storage=2)
+    c = H.HipVolume(0.01, 1024, 0.04, 32.0, storage=1)
+    r = H.HipVolume(0.01, 1024, 0.04, 32.0)
+    with pytest.raises(RuntimeError):
+        H.Batch([a, c])       # mixed storage
+    with pytest.raises(RuntimeError):
+        H.Batch([r])          # weighted-sum storage has no projective update
+    with pytest.raises(RuntimeError):
+        H.Batch([a, a])       # the same volume twice
+    b = H.Batch([a])
+    with pytest.raises(RuntimeError):
+        H.Batch([a])          # already in a batch
+    fr = H.make_frame(8, 8, 4, 4, 4, 4, np.eye(3), np.zeros(3))
+    d = torch.ones(64, dtype=torch.float32, device="cuda")
+    assert a.integrate_depth(d, fr) == 3  # single calls refused while batched
+    b.close()
+    assert a.integrate_depth(d, fr) == 0

@@ -1,0 +1,93 @@
+"""CPU checks of host-side logic and of the safety argument behind the device
+band walk's cull shortcut."""
+import math
+
+import numpy as np
+import pytest
+
+f32 = np.float32
+
+
+def band_walk_f32(p, cam, voxel, trunc, steps):
+    """The reference band walk (cpp/src/volume.cpp:158-191) evaluated in numpy
+    float32, one rounded operation at a time: per sample, (culled, bound)
+    where bound is the device's shortcut bound."""
+    p = p.astype(f32)
+    cam = cam.astype(f32)
+    voxel = f32(voxel)
+    trunc = f32(trunc)
+    inv_voxel = f32(1.0) / voxel
+    d = p - cam
+    d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+    dist = np.sqrt(d2)
+    u = d / dist[:, None]
+    slack = f32(7.63e-6) * (np.abs(p).sum(1, dtype=f32) + np.abs(cam).sum(dtype=f32) + dist +
+                            f32(steps) * voxel)
+    out = []
+    for s in range(-steps, steps + 1):
+        t = f32(s) * voxel
+        v = np.floor((p + u * t) * inv_voxel)
+        c = (v + f32(0.5)) * voxel
+        e = c - cam
+        sdf = dist - np.sqrt(e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1] + e[:, 2] * e[:, 2])
+        culled = sdf < -trunc
+        bnd = np.where(dist + t >= 0, t, np.abs(t)) + f32(0.8661) * voxel + slack
+        out.append((culled, bnd))
+    return out, trunc
+
+
+@pytest.mark.parametrize("voxel,trunc,scale", [
+    (0.01, 0.04, 5.0), (0.02, 0.08, 8.0), (0.005, 0.02, 12.0), (0.01, 0.04, 100.0),
+    (0.01, 0.025, 3.0), (0.05, 0.11, 30.0),
+])
+def test_cull_shortcut_never_hides_a_cull(voxel, trunc, scale):
+    """Whenever the device skips the square root (bound < trunc), the
+    reference expression would not have culled the sample either."""
+    rng = np.random.default_rng(int(voxel * 1e4) + int(scale))
+    n = 200000
+    cam = rng.uniform(-scale, scale, 3)
+    # points at all ranges, including inside the band (dist < trunc)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    r = np.concatenate([rng.uniform(0.0005, 2 * trunc, n // 4), rng.uniform(0.1, scale, n - n // 4)])
+    p = cam + dirs * r[:, None]
+    steps = max(1, math.ceil(np.float32(trunc) * (np.float32(1.0) / np.float32(voxel))))
+    samples, tr = band_walk_f32(p, cam, voxel, trunc, steps)
+    skipped = 0
+    for culled, bnd in samples:
+        skip = bnd < tr
+        assert not np.any(culled & skip)
+        skipped += int(skip.sum())
+    assert skipped > 0
+
+
+def test_trajectory_is_smooth_and_inside_room():
+    import scenes as S
+    scene, poses = S.build(dict(S.PRECISE, frames=300), 3)
+    lo, hi = scene.room_lo, scene.room_hi
+    prev = None
+    for R, t in poses:
+        assert np.allclose(R.T @ R, np.eye(3), atol=1e-12)
+        assert abs(np.linalg.det(R) - 1.0) < 1e-12
+        assert np.all(t > lo + 0.3) and np.all(t < hi - 0.3)
+        if prev is not None:
+            step = np.linalg.norm(t - prev[1])
+            assert abs(step - 0.02) < 1e-3
+            ang = math.degrees(math.acos(np.clip((np.trace(prev[0].T @ R) - 1) / 2, -1, 1)))
+            assert 0.5 < ang < 1.5
+        prev = (R, t)
+    # primitives keep clear of the camera path
+    for sp in scene.spheres:
+        d = np.linalg.norm([np.hypot(*(sp[:2] - scene.path_centre[:2])) - scene.path_radius,
+                            sp[2] - scene.path_centre[2]])
+        assert d > sp[3]
+
+
+def test_frame_descriptor_casts_once():
+    import osn_tsdf_hip as H
+    R = np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]]) * (1 + 1e-9)
+    t = np.array([0.1, 0.2, 1.0 / 3.0])
+    f = H.make_frame(640, 480, 320, 320, 320, 240, R, t)
+    assert list(f.t_wc64) == list(t)
+    assert np.float32(f.t_wc[2]) == np.float32(1.0 / 3.0)
+    assert f.width == 640 and f.height == 480

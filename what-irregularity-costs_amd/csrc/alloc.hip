@@ -12,6 +12,10 @@
 // index, so there is no spin, and phase B assigns indices in key order, which
 // makes the pool order a function of the key sets alone.
 //
+// Appends to the call's touched/new lists are buffered in LDS and published
+// with one global atomic per workgroup: per-wave atomics on one counter
+// serialise in L2 and were measured to dominate the kernel.
+//
 // Compiled with -ffp-contract=off: the band walk must round exactly as the
 // reference's x86-64 g++ build (SSE, no FMA) so that the allocated key set is
 // bit-identical.
@@ -23,6 +27,10 @@ namespace {
 
 constexpr int kAllocThreads = 256;  // 4 waves; 16x16 pixels in depth mode
 constexpr int kKeysPerLane = 8;     // distinct keys a lane buffers before a flush
+constexpr int kWgList = 1024;       // LDS list capacity per workgroup
+constexpr int kMaxSet = 1024;       // LDS pre-hash capacity
+
+enum SrcKind : int { kSrcPoints = 0, kSrcDepth = 1, kSrcCoords = 2 };
 
 __device__ inline int64_t readlane64(int64_t x, int lane) {
     const int lo = __builtin_amdgcn_readlane(int(uint32_t(uint64_t(x))), lane);
@@ -35,23 +43,53 @@ __device__ inline unsigned long long lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Append `mine` (when `want`) to a device list behind a counter with one
-// atomic per wave. Must be called by the whole wave.
+__device__ inline unsigned long long wave_sum_u64(unsigned long long x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+struct WgShared {
+    int64_t set[kMaxSet];
+    int64_t new_keys[kWgList];
+    uint32_t touched[kWgList];
+    int32_t n_new, n_touched;
+    int32_t base_new, base_touched;
+    unsigned long long drops, cas, lookups, lane_steps, wave_steps;
+    unsigned long long hist[kProbeHistBins];
+};
+
+// Append `mine` (when `want`) to an LDS list, one LDS atomic per wave; past
+// the LDS capacity, straight to the global list.
 template <typename T>
-__device__ inline void wave_append(bool want, T mine, int32_t* counter, T* list) {
+__device__ inline void wave_append(bool want, T mine, int32_t* lds_count, T* lds_list,
+                                   int32_t* g_count, T* g_list) {
     const unsigned long long m = __ballot(want);
     if (m == 0) return;
     const int leader = __ffsll((long long)m) - 1;
     int base = 0;
-    if (__lane_id() == leader) base = atomicAdd(counter, __popcll(m));
+    if (__lane_id() == leader) base = atomicAdd(lds_count, __popcll(m));
     base = __builtin_amdgcn_readlane(base, leader);
-    if (want) list[base + __popcll(m & lanemask_lt())] = mine;
+    if (!want) return;
+    const int i = base + __popcll(m & lanemask_lt());
+    if (i < kWgList) lds_list[i] = mine;
+    else g_list[atomicAdd(g_count, 1)] = mine;
 }
 
-__device__ inline void wave_add_u64(unsigned long long x, unsigned long long* dst) {
-    // reduce across the wave, one atomic
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    if (__lane_id() == 0 && x) atomicAdd(dst, x);
+// Publish the workgroup's buffered lists: one global atomic per list.
+__device__ void flush_lists(const VolDev& v, ListCounters* lc, WgShared& s) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nt = min(s.n_touched, kWgList), nn = min(s.n_new, kWgList);
+        s.base_touched = nt ? atomicAdd(&lc->touched_count, nt) : 0;
+        s.base_new = nn ? atomicAdd(&lc->new_count, nn) : 0;
+    }
+    __syncthreads();
+    const int nt = min(s.n_touched, kWgList), nn = min(s.n_new, kWgList);
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) v.touched[s.base_touched + i] = s.touched[i];
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) v.new_keys[s.base_new + i] = s.new_keys[i];
+    __syncthreads();
+    if (threadIdx.x == 0) { s.n_touched = 0; s.n_new = 0; }
+    __syncthreads();
 }
 
 struct LaneKeys {
@@ -59,12 +97,16 @@ struct LaneKeys {
     uint32_t valid = 0;
     int64_t last = kEmptyKey;
 
+    // Buffer a key unless it repeats the previous one. A lane walks one ray,
+    // and a ray meets each (convex) block in one contiguous run of samples, so
+    // comparing with the previous key removes every in-lane repeat.
     __device__ inline void push(int64_t key) {
+        if (key == last) return;
+        last = key;
 #pragma unroll
         for (int j = kKeysPerLane - 1; j > 0; --j) k[j] = k[j - 1];
         k[0] = key;
         valid = ((valid << 1) | 1u) & ((1u << kKeysPerLane) - 1u);
-        last = key;
     }
     __device__ inline int64_t first() const {
         int64_t f = kEmptyKey;
@@ -80,39 +122,39 @@ struct LaneKeys {
     }
 };
 
-struct WgState {
-    int64_t* set;     // LDS pre-hash, `set_size` entries
-    int set_size;
-    unsigned long long* hist;  // LDS histogram (probe mode)
+struct TileCtx {
+    const VolDev* v;
+    ListCounters* lc;
+    WgShared* s;
+    int32_t set_size;
     int32_t stamp;
 };
 
 // One batch of up to 64 distinct keys, one per lane (has = lane holds one).
 template <int MODE, bool TOUCH>
-__device__ void process_keys(const VolDev& v, const WgState& wg, bool has, int64_t key,
-                             unsigned long long& cas_acc, unsigned long long& drop_acc,
-                             unsigned long long& lane_acc, unsigned long long& lookup_acc,
-                             unsigned long long& wave_acc) {
+__device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
+    const VolDev& v = *c.v;
+    WgShared& s = *c.s;
     // Workgroup pre-hash: only the lane that claims a key in LDS goes on to the
-    // global table; the other waves of the workgroup drop their copy.
+    // global table; copies from the other waves of the workgroup stop here.
     bool go = has;
-    if (has && wg.set_size > 0) {
-        uint32_t h = uint32_t((uint64_t(key) * 0x9E3779B97F4A7C15ull) >> 40) & (wg.set_size - 1);
+    if (has && c.set_size > 0) {
+        uint32_t h = uint32_t((uint64_t(key) * 0x9E3779B97F4A7C15ull) >> 40) & (c.set_size - 1);
         for (int p = 0; p < 8; ++p) {
-            const unsigned long long prev = atomicCAS(
-                reinterpret_cast<unsigned long long*>(&wg.set[h]),
-                (unsigned long long)kEmptyKey, (unsigned long long)key);
+            const unsigned long long prev =
+                atomicCAS(reinterpret_cast<unsigned long long*>(&s.set[h]),
+                          (unsigned long long)kEmptyKey, (unsigned long long)key);
             if (prev == (unsigned long long)kEmptyKey) break;  // ours
             if (prev == (unsigned long long)key) { go = false; break; }
-            h = (h + 1) & (wg.set_size - 1);
+            h = (h + 1) & (c.set_size - 1);
         }
     }
 
     bool inserted = false, found = false;
     uint32_t slot = 0;
     uint32_t steps = 0;
+    unsigned long long cas = 0, drops = 0;
     if (go) {
-        lookup_acc += 1;
         slot = home_slot(v, key);
         for (uint32_t p = 0; p < v.probe_limit; ++p) {
             ++steps;
@@ -121,7 +163,7 @@ __device__ void process_keys(const VolDev& v, const WgState& wg, bool has, int64
             if (cur == key) { found = true; break; }
             if (cur == kEmptyKey) {
                 if (MODE == kAllocNoCas) break;
-                if (MODE == kAllocCountCas) cas_acc += 1;
+                if (MODE == kAllocCountCas) cas += 1;
                 const unsigned long long prev =
                     atomicCAS(reinterpret_cast<unsigned long long*>(&v.table[slot].key),
                               (unsigned long long)kEmptyKey, (unsigned long long)key);
@@ -130,44 +172,94 @@ __device__ void process_keys(const VolDev& v, const WgState& wg, bool has, int64
             }
             slot = next_slot(v, slot);
         }
-        if (MODE != kAllocNoCas && !found && !inserted) drop_acc += 1;  // probe bound hit
+        if (MODE != kAllocNoCas && !found && !inserted) drops += 1;  // probe bound hit
         if (found) {
-            const int32_t idx =
-                __hip_atomic_load(&v.table[slot].block_idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (idx == kIdxDropped) drop_acc += 1;
+            const int32_t idx = __hip_atomic_load(&v.table[slot].block_idx, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+            if (idx == kIdxDropped) drops += 1;
         }
     }
 
+    // per-workgroup statistics (rare or measurement-only), one wave-level add
+    const unsigned long long wd = wave_sum_u64(drops);
+    if (__lane_id() == 0 && wd) atomicAdd(&s.drops, wd);
+    if (MODE == kAllocCountCas) {
+        const unsigned long long wc = wave_sum_u64(cas);
+        if (__lane_id() == 0 && wc) atomicAdd(&s.cas, wc);
+    }
     if (MODE == kAllocProbe) {
-        lane_acc += steps;
-        if (go) {
-            const int bin = steps < (uint32_t)kProbeHistBins ? int(steps) - 1 : kProbeHistBins - 1;
-            atomicAdd(&wg.hist[bin], 1ull);
-        }
-        // divergence cost: the wave runs the probe loop until its deepest lane
-        // finishes, for every lane
+        const unsigned long long ws = wave_sum_u64(steps);
+        const unsigned long long wl = wave_sum_u64(go ? 1ull : 0ull);
         uint32_t mx = steps;
         for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
-        if (__lane_id() == 0) wave_acc += 64ull * mx;
+        if (__lane_id() == 0) {
+            atomicAdd(&s.lane_steps, ws);
+            atomicAdd(&s.lookups, wl);
+            atomicAdd(&s.wave_steps, 64ull * mx);  // the wave runs until its deepest lane is done
+        }
+        if (go) {
+            const int bin = steps < (uint32_t)kProbeHistBins ? int(steps) - 1 : kProbeHistBins - 1;
+            atomicAdd(&s.hist[bin], 1ull);
+        }
     }
 
     bool touch = false;
     if (TOUCH && (found || inserted)) {
-        const int32_t old = atomicExch(&v.table[slot].stamp, wg.stamp);
-        touch = (old != wg.stamp);
+        const int32_t old = atomicExch(&v.table[slot].stamp, c.stamp);
+        touch = (old != c.stamp);
     }
-    if (TOUCH) wave_append<uint32_t>(touch, slot, &v.ctr->touched_count, v.touched);
-    if (MODE != kAllocNoCas) wave_append<int64_t>(inserted, key, &v.ctr->new_count, v.new_keys);
+    if (TOUCH)
+        wave_append<uint32_t>(touch, slot, &s.n_touched, s.touched, &c.lc->touched_count, v.touched);
+    if (MODE != kAllocNoCas)
+        wave_append<int64_t>(inserted, key, &s.n_new, s.new_keys, &c.lc->new_count, v.new_keys);
 }
 
 // Drain every lane's buffered keys: repeatedly take the first live key of the
 // first live lane, strike it from every lane, and hand it to lane `nd`. Each
 // distinct key of the wave is therefore processed once, 64 at a time.
+//
+// Most keys of a wave lie in a few neighbouring blocks, so a window pass goes
+// first: every lane marks its keys that fall in the 4x4x4 blocks starting one
+// block below the first live key, the 64-bit marks are OR-ed across the wave,
+// and lane i takes window cell i. Only keys outside the window reach the
+// one-key-per-iteration loop.
+__device__ inline unsigned long long wave_or_u64(unsigned long long x) {
+    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
+    return x;
+}
+
 template <int MODE, bool TOUCH>
-__device__ void flush_wave(const VolDev& v, const WgState& wg, LaneKeys& lk,
-                           unsigned long long& cas_acc, unsigned long long& drop_acc,
-                           unsigned long long& lane_acc, unsigned long long& lookup_acc,
-                           unsigned long long& wave_acc) {
+__device__ void flush_wave(const TileCtx& c, LaneKeys& lk) {
+    const unsigned long long live = __ballot(lk.valid != 0);
+    if (live == 0) return;
+    {
+        const int leader = __ffsll((long long)live) - 1;
+        const uint64_t k0 = (uint64_t)readlane64(lk.first(), leader);
+        constexpr uint64_t kM = (uint64_t(1) << kCoordBits) - 1;
+        // biased coordinates of the window origin (wraps harmlessly at 0)
+        const uint32_t ox = uint32_t(k0 >> (2 * kCoordBits)) - 1u;
+        const uint32_t oy = uint32_t((k0 >> kCoordBits) & kM) - 1u;
+        const uint32_t oz = uint32_t(k0 & kM) - 1u;
+        unsigned long long cells = 0;
+#pragma unroll
+        for (int j = 0; j < kKeysPerLane; ++j) {
+            if (!((lk.valid >> j) & 1u)) continue;
+            const uint64_t key = (uint64_t)lk.k[j];
+            const uint32_t dx = uint32_t(key >> (2 * kCoordBits)) - ox;
+            const uint32_t dy = uint32_t((key >> kCoordBits) & kM) - oy;
+            const uint32_t dz = uint32_t(key & kM) - oz;
+            if ((dx | dy | dz) < 4u) {
+                cells |= 1ull << (dx | (dy << 2) | (dz << 4));
+                lk.valid &= ~(1u << j);
+            }
+        }
+        cells = wave_or_u64(cells);
+        const int lane = __lane_id();
+        const uint64_t mine = (uint64_t(ox + (lane & 3)) << (2 * kCoordBits)) |
+                              (uint64_t(oy + ((lane >> 2) & 3)) << kCoordBits) |
+                              uint64_t(oz + (lane >> 4));
+        process_keys<MODE, TOUCH>(c, (cells >> lane) & 1ull, (int64_t)mine);
+    }
     int nd = 0;
     int64_t mine = kEmptyKey;
     for (;;) {
@@ -178,50 +270,35 @@ __device__ void flush_wave(const VolDev& v, const WgState& wg, LaneKeys& lk,
         lk.remove(k);
         if (__lane_id() == nd) mine = k;
         if (++nd == 64) {
-            process_keys<MODE, TOUCH>(v, wg, true, mine, cas_acc, drop_acc, lane_acc, lookup_acc,
-                                      wave_acc);
+            process_keys<MODE, TOUCH>(c, true, mine);
             nd = 0;
             mine = kEmptyKey;
         }
     }
-    if (nd > 0)
-        process_keys<MODE, TOUCH>(v, wg, __lane_id() < nd, mine, cas_acc, drop_acc, lane_acc,
-                                  lookup_acc, wave_acc);
+    if (nd > 0) process_keys<MODE, TOUCH>(c, __lane_id() < nd, mine);
 }
 
-// One point per thread. FROM_DEPTH: a 16x16 pixel tile per workgroup, 8x8 per
-// wave; the point is the pixel back-projected in FP32 as
+// One tile: 256 points (points / coords) or 16x16 pixels (depth), one per
+// thread. A depth pixel is back-projected in FP32 as
 //   xc = (u - cx) * z / fx, yc = (v - cy) * z / fy, p = R * (xc, yc, z) + cam
 // evaluated left to right without fused multiply-add.
-enum SrcKind : int { kSrcPoints = 0, kSrcDepth = 1, kSrcCoords = 2 };
-
 template <int MODE, bool TOUCH, int SRC>
-__global__ __launch_bounds__(kAllocThreads) void k_alloc(VolDev v, PointSrc src, const int32_t* coords,
-                                                         int32_t set_size) {
-    __shared__ int64_t s_set[2048];
-    __shared__ unsigned long long s_hist[kProbeHistBins];
-    for (int i = threadIdx.x; i < set_size; i += kAllocThreads) s_set[i] = kEmptyKey;
-    if (MODE == kAllocProbe)
-        for (int i = threadIdx.x; i < kProbeHistBins; i += kAllocThreads) s_hist[i] = 0;
-    __syncthreads();
-
-    WgState wg{s_set, set_size, s_hist, __hip_atomic_load(&v.ctr->stamp, __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT)};
-
-    unsigned long long cas_acc = 0, drop_acc = 0, lane_acc = 0, lookup_acc = 0, wave_acc = 0;
+__device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t* coords, int tile) {
+    const VolDev& v = *c.v;
     LaneKeys lk;
     if (SRC == kSrcCoords) {
-        // allocator stress path: the keys are given, no band walk
-        const int i = blockIdx.x * kAllocThreads + threadIdx.x;
+        const int i = tile * kAllocThreads + threadIdx.x;
         if (i < src.n) lk.push(pack_coord(coords[i * 3 + 0], coords[i * 3 + 1], coords[i * 3 + 2]));
-        flush_wave<MODE, TOUCH>(v, wg, lk, cas_acc, drop_acc, lane_acc, lookup_acc, wave_acc);
-    } else {
+        flush_wave<MODE, TOUCH>(c, lk);
+        return;
+    }
     bool ok = false;
     float px = 0.f, py = 0.f, pz = 0.f;
     if (SRC == kSrcDepth) {
+        const int tiles_x = (src.W + 15) >> 4;
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int u = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-        const int w = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+        const int u = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+        const int w = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
         if (u < src.W && w < src.H) {
             const float z = src.depth[(size_t)w * src.W + u];
             ok = (z > 0.0f) && __builtin_isfinite(z) && !(src.max_range > 0.0f && z > src.max_range);
@@ -234,7 +311,7 @@ __global__ __launch_bounds__(kAllocThreads) void k_alloc(VolDev v, PointSrc src,
             }
         }
     } else {
-        const int i = blockIdx.x * kAllocThreads + threadIdx.x;
+        const int i = tile * kAllocThreads + threadIdx.x;
         if (i < src.n) {
             px = src.pos[i * 3 + 0];
             py = src.pos[i * 3 + 1];
@@ -244,7 +321,7 @@ __global__ __launch_bounds__(kAllocThreads) void k_alloc(VolDev v, PointSrc src,
     }
 
     // Band walk (volume.cpp:158-191).
-    float dist = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+    float dist = 0.f, ux = 0.f, uy = 0.f, uz = 0.f, slack = 0.f;
     if (ok) {
         const float dx = px - src.cam[0], dy = py - src.cam[1], dz = pz - src.cam[2];
         const float d2 = dx * dx + dy * dy + dz * dz;
@@ -252,53 +329,142 @@ __global__ __launch_bounds__(kAllocThreads) void k_alloc(VolDev v, PointSrc src,
         dist = sqrtf(d2);
         if (!(dist > 1e-6f)) ok = false;
         ux = dx / dist; uy = dy / dist; uz = dz / dist;
+        // Cull shortcut. The voxel centre chosen for a sample at signed offset
+        // t along the ray lies within half a voxel diagonal of the sample, so
+        // in exact arithmetic sdf >= -(B + 0.8661 voxel) with B = t when the
+        // sample is on the camera's side of the point's far end (dist + t >= 0)
+        // and |t| otherwise. Every FP32 rounding on the way is below 2^-23 of
+        // the magnitudes involved, a few dozen times over; `slack` bounds them
+        // all with a 64 * 2^-23 relative margin. A sample whose bound clears
+        // -trunc by more than the slack cannot be culled, so its square root
+        // is skipped; the others evaluate the reference's expression as is.
+        slack = 7.63e-6f * (fabsf(px) + fabsf(py) + fabsf(pz) + fabsf(src.cam[0]) +
+                            fabsf(src.cam[1]) + fabsf(src.cam[2]) + dist +
+                            (float)v.steps * v.voxel);
     }
-
     for (int32_t s = -v.steps; s <= v.steps; ++s) {
         if (ok) {
             const float t = (float)s * v.voxel;
             const int32_t vx = (int32_t)floorf((px + ux * t) * v.inv_voxel);
             const int32_t vy = (int32_t)floorf((py + uy * t) * v.inv_voxel);
             const int32_t vz = (int32_t)floorf((pz + uz * t) * v.inv_voxel);
-            const float cx = ((float)vx + 0.5f) * v.voxel;
-            const float cy = ((float)vy + 0.5f) * v.voxel;
-            const float cz = ((float)vz + 0.5f) * v.voxel;
-            const float ex = cx - src.cam[0], ey = cy - src.cam[1], ez = cz - src.cam[2];
-            const float sdf = dist - sqrtf(ex * ex + ey * ey + ez * ez);
-            if (!(sdf < -v.trunc)) {
-                const int64_t key = pack_coord(floor_div8(vx), floor_div8(vy), floor_div8(vz));
-                if (key != lk.last) lk.push(key);
+            const float bound = (dist + t >= 0.0f ? t : fabsf(t)) + 0.8661f * v.voxel + slack;
+            bool culled = false;
+            if (bound >= v.trunc) {
+                const float cx = ((float)vx + 0.5f) * v.voxel;
+                const float cy = ((float)vy + 0.5f) * v.voxel;
+                const float cz = ((float)vz + 0.5f) * v.voxel;
+                const float ex = cx - src.cam[0], ey = cy - src.cam[1], ez = cz - src.cam[2];
+                const float sdf = dist - sqrtf(ex * ex + ey * ey + ez * ez);
+                culled = sdf < -v.trunc;
             }
+            if (!culled) lk.push(pack_coord(floor_div8(vx), floor_div8(vy), floor_div8(vz)));
         }
-        if (__ballot(lk.valid >> (kKeysPerLane - 1)))
-            flush_wave<MODE, TOUCH>(v, wg, lk, cas_acc, drop_acc, lane_acc, lookup_acc, wave_acc);
+        if (__ballot(lk.valid >> (kKeysPerLane - 1))) flush_wave<MODE, TOUCH>(c, lk);
     }
-    flush_wave<MODE, TOUCH>(v, wg, lk, cas_acc, drop_acc, lane_acc, lookup_acc, wave_acc);
-    }
+    flush_wave<MODE, TOUCH>(c, lk);
+}
 
-    // the reference's counting variant tallies attempts into drop_count
-    // (cpp/src/volume_cuda.cu:59-61,81); keep a separate total as well
-    if (MODE == kAllocCountCas) {
-        wave_add_u64(cas_acc, &v.ctr->cas_attempts);
-        wave_add_u64(cas_acc, &v.ctr->drop_count);
-    }
-    wave_add_u64(drop_acc, &v.ctr->drop_count);
-    wave_add_u64(lookup_acc, &v.ctr->lookups);
-    if (MODE == kAllocProbe) {
-        wave_add_u64(lane_acc, &v.ctr->lane_steps);
-        if (__lane_id() == 0 && wave_acc) atomicAdd(&v.ctr->wave_steps, wave_acc);
-        __syncthreads();
-        for (int i = threadIdx.x; i < kProbeHistBins; i += kAllocThreads)
-            if (s_hist[i]) atomicAdd(&v.ctr->probe_hist[i], s_hist[i]);
+__device__ void wg_init(WgShared& s, int set_size) {
+    for (int i = threadIdx.x; i < set_size; i += blockDim.x) s.set[i] = kEmptyKey;
+    for (int i = threadIdx.x; i < kProbeHistBins; i += blockDim.x) s.hist[i] = 0;
+    if (threadIdx.x == 0) {
+        s.n_new = s.n_touched = 0;
+        s.drops = s.cas = s.lookups = s.lane_steps = s.wave_steps = 0;
     }
 }
 
-// Start of an allocate call: new stamp, empty per-call lists. A kernel rather
-// than a memset so that a captured graph replays it correctly.
-__global__ void k_begin_call(Counters* c) {
-    c->stamp = c->stamp + 1;
-    c->new_count = 0;
-    c->touched_count = 0;
+__device__ void clear_set(WgShared& s, int set_size) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < set_size; i += blockDim.x) s.set[i] = kEmptyKey;
+    __syncthreads();
+}
+
+template <int MODE>
+__device__ void flush_stats(const VolDev& v, WgShared& s) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s.drops) atomicAdd(&v.ctr->drop_count, s.drops);
+        if (MODE == kAllocCountCas && s.cas) {
+            // the reference's counting variant tallies attempts into
+            // drop_count (cpp/src/volume_cuda.cu:59-61,81); keep a total too
+            atomicAdd(&v.ctr->cas_attempts, s.cas);
+            atomicAdd(&v.ctr->drop_count, s.cas);
+        }
+        if (MODE == kAllocProbe) {
+            atomicAdd(&v.ctr->lane_steps, s.lane_steps);
+            atomicAdd(&v.ctr->wave_steps, s.wave_steps);
+            atomicAdd(&v.ctr->lookups, s.lookups);
+        }
+        s.drops = s.cas = s.lookups = s.lane_steps = s.wave_steps = 0;
+    }
+    if (MODE == kAllocProbe)
+        for (int i = threadIdx.x; i < kProbeHistBins; i += blockDim.x) {
+            if (s.hist[i]) atomicAdd(&v.ctr->probe_hist[i], s.hist[i]);
+            s.hist[i] = 0;
+        }
+    __syncthreads();
+}
+
+__device__ inline void reset_other_parity(Counters* ctr, int parity) {
+    ctr->lists[parity ^ 1].new_count = 0;
+    ctr->lists[parity ^ 1].touched_count = 0;
+}
+
+template <int MODE, bool TOUCH, int SRC>
+__global__ __launch_bounds__(kAllocThreads) void k_alloc(VolDev v, PointSrc src,
+                                                         const int32_t* coords, int32_t set_size,
+                                                         CallInfo call, int32_t n_tiles) {
+    __shared__ WgShared s;
+    wg_init(s, set_size);
+    if (blockIdx.x == 0 && threadIdx.x == 0) reset_other_parity(v.ctr, call.parity);
+    __syncthreads();
+    const TileCtx c{&v, &v.ctr->lists[call.parity], &s, set_size, call.stamp};
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        if (tile != (int)blockIdx.x) clear_set(s, set_size);
+        alloc_tile<MODE, TOUCH, SRC>(c, src, coords, tile);
+    }
+    flush_lists(v, c.lc, s);
+    flush_stats<MODE>(v, s);
+}
+
+// Many scenes in one launch: tiles of all active scenes form one work list;
+// each scene keeps its own table and lists, flushed when the workgroup moves
+// on to another scene.
+__global__ __launch_bounds__(kAllocThreads) void k_alloc_batch(BatchDev b, int32_t set_size,
+                                                               int32_t total_tiles) {
+    __shared__ WgShared s;
+    wg_init(s, set_size);
+    __syncthreads();
+    const int k = b.step_ctr ? *b.step_ctr : b.step;
+    int cur = -1;
+    CallInfo call{0, 0};
+    VolDev v{};
+    for (int tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
+        int sc = 0;
+        while (sc + 1 < b.n_scenes && b.tile_prefix[sc + 1] <= tile) ++sc;
+        const SceneStep& st = b.steps[k * b.n_scenes + sc];
+        const int local = tile - b.tile_prefix[sc];
+        if (local == 0 && threadIdx.x == 0)
+            reset_other_parity(b.vols[sc].ctr, batch_call(b, sc, k).parity);
+        if (!st.active || local >= st.tiles) continue;
+        if (sc != cur) {
+            if (cur >= 0) {
+                flush_lists(v, &v.ctr->lists[call.parity], s);
+                flush_stats<kAllocNormal>(v, s);
+            }
+            cur = sc;
+            v = b.vols[sc];
+            call = batch_call(b, sc, k);
+        }
+        clear_set(s, set_size);
+        const TileCtx c{&v, &v.ctr->lists[call.parity], &s, set_size, call.stamp};
+        alloc_tile<kAllocNormal, true, kSrcDepth>(c, st.src, nullptr, local);
+    }
+    if (cur >= 0) {
+        flush_lists(v, &v.ctr->lists[call.parity], s);
+        flush_stats<kAllocNormal>(v, s);
+    }
 }
 
 // ---- phase B: sort the call's new keys, assign pool indices in key order ----
@@ -333,20 +499,25 @@ __device__ void merge_runs(const int64_t* a, int la, const int64_t* b, int lb, i
     }
 }
 
-__global__ __launch_bounds__(1024) void k_assign(VolDev v, int32_t flag_stride) {
-    __shared__ int64_t s_keys[kSortTile];
-    __shared__ int s_arrival;
-    const int n = v.ctr->new_count;
-    const int tile = blockIdx.x;
-    if (n == 0 || tile * kSortTile >= n) return;
-
-    // 1. bitonic sort of this tile in LDS, padded to a power of two
-    const int lo = tile * kSortTile;
-    const int cnt = min(kSortTile, n - lo);
+// Sort s_keys[0, cnt) in LDS: rank sort for short lists (one pass, one
+// barrier), bitonic sort over the padded power of two otherwise. Keys are
+// distinct. Result in s_out (rank sort) or s_keys (bitonic); returns the array.
+__device__ int64_t* lds_sort(int64_t* s_keys, int64_t* s_out, int cnt) {
+    if (cnt <= (int)blockDim.x) {
+        int64_t mine = 0;
+        int rank = 0;
+        if ((int)threadIdx.x < cnt) {
+            mine = s_keys[threadIdx.x];
+            for (int j = 0; j < cnt; ++j) rank += s_keys[j] < mine;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) s_out[rank] = mine;
+        __syncthreads();
+        return s_out;
+    }
     int P = 2;
     while (P < cnt) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += blockDim.x)
-        s_keys[i] = i < cnt ? v.new_keys[lo + i] : INT64_MAX;
+    for (int i = cnt + threadIdx.x; i < P; i += blockDim.x) s_keys[i] = INT64_MAX;
     __syncthreads();
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -361,47 +532,14 @@ __global__ __launch_bounds__(1024) void k_assign(VolDev v, int32_t flag_stride) 
             __syncthreads();
         }
     }
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) v.sort_a[lo + i] = s_keys[i];
+    return s_keys;
+}
 
-    // 2. merge tree: the second of two sibling runs to finish merges them, so
-    //    no workgroup ever waits on another
-    int64_t* src = v.sort_a;
-    int64_t* dst = v.sort_b;
-    int run = tile;
-    for (int level = 0; (kSortTile << level) < n; ++level) {
-        const int width = kSortTile << level;
-        const int pair = run >> 1;
-        const int a_lo = pair * 2 * width;
-        const int a_hi = min(a_lo + width, n);
-        const int b_hi = min(a_lo + 2 * width, n);
-        if (a_hi >= b_hi) {
-            for (int i = a_lo + (int)threadIdx.x; i < a_hi; i += blockDim.x) dst[i] = src[i];
-        } else {
-            __threadfence();
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                int* flag = &v.sort_flags[level * flag_stride + pair];
-                const int old = atomicAdd(flag, 1);
-                if (old == 1) *flag = 0;  // reset for the next call
-                s_arrival = old;
-            }
-            __syncthreads();
-            if (s_arrival == 0) return;  // sibling not done; it will merge
-            __threadfence();
-            merge_runs(src + a_lo, a_hi - a_lo, src + a_hi, b_hi - a_hi, dst + a_lo);
-        }
-        __syncthreads();
-        int64_t* t = src; src = dst; dst = t;
-        run = pair;
-    }
-    __threadfence();
-    __syncthreads();
-
-    // 3. this workgroup holds all n keys in order: assign pool indices
+__device__ void assign_sorted(const VolDev& v, const int64_t* keys, int n) {
     const int bc0 = v.ctr->block_count;
     unsigned long long drops = 0;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        const int64_t key = src[r];
+        const int64_t key = keys[r];
         const uint32_t slot = find_slot(v, key);
         if (slot == 0xffffffffu) continue;
         const int64_t idx = (int64_t)bc0 + r;
@@ -417,9 +555,88 @@ __global__ __launch_bounds__(1024) void k_assign(VolDev v, int32_t flag_stride) 
             drops += 1;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) drops += __shfl_xor(drops, o);
+    drops = wave_sum_u64(drops);
     if ((threadIdx.x & 63) == 0 && drops) atomicAdd(&v.ctr->drop_count, drops);
-    if (threadIdx.x == 0) v.ctr->block_count = (int32_t)min((int64_t)bc0 + n, (int64_t)v.pool_capacity);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        v.ctr->block_count = (int32_t)min((int64_t)bc0 + n, (int64_t)v.pool_capacity);
+}
+
+// Workgroups stride over 4096-key tiles; each sorts its tile in LDS, then
+// climbs a merge tree in which the second of two sibling runs to finish merges
+// them, so no workgroup ever waits on another. The one that completes the top
+// assigns indices bc0, bc0+1, ... in key order.
+__device__ void assign_volume(const VolDev& v, int parity, int32_t flag_stride, int wg, int n_wg,
+                              int64_t* s_keys, int64_t* s_tmp) {
+    __shared__ int s_arrival;
+    const int n = v.ctr->lists[parity].new_count;
+    if (n == 0) return;
+    const int n_tiles = (n + kSortTile - 1) / kSortTile;
+    for (int tile = wg; tile < n_tiles; tile += n_wg) {
+        const int lo = tile * kSortTile;
+        const int cnt = min(kSortTile, n - lo);
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) s_keys[i] = v.new_keys[lo + i];
+        __syncthreads();
+        int64_t* sorted = lds_sort(s_keys, s_tmp, cnt);
+        if (n_tiles == 1) {  // the common case: everything stays in LDS
+            assign_sorted(v, sorted, cnt);
+            return;
+        }
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) v.sort_a[lo + i] = sorted[i];
+        int64_t* src = v.sort_a;
+        int64_t* dst = v.sort_b;
+        int run = tile;
+        bool top = true;
+        for (int level = 0; (kSortTile << level) < n; ++level) {
+            const int width = kSortTile << level;
+            const int pair = run >> 1;
+            const int a_lo = pair * 2 * width;
+            const int a_hi = min(a_lo + width, n);
+            const int b_hi = min(a_lo + 2 * width, n);
+            if (a_hi >= b_hi) {
+                for (int i = a_lo + (int)threadIdx.x; i < a_hi; i += blockDim.x) dst[i] = src[i];
+            } else {
+                __threadfence();
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    int* flag = &v.sort_flags[level * flag_stride + pair];
+                    const int old = atomicAdd(flag, 1);
+                    if (old == 1) *flag = 0;  // reset for the next call
+                    s_arrival = old;
+                }
+                __syncthreads();
+                if (s_arrival == 0) { top = false; break; }  // sibling will merge
+                __threadfence();
+                merge_runs(src + a_lo, a_hi - a_lo, src + a_hi, b_hi - a_hi, dst + a_lo);
+            }
+            __syncthreads();
+            int64_t* t = src; src = dst; dst = t;
+            run = pair;
+        }
+        if (top) {
+            __threadfence();
+            __syncthreads();
+            assign_sorted(v, src, n);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_assign(VolDev v, CallInfo call, int32_t flag_stride) {
+    __shared__ int64_t s_keys[kSortTile];
+    __shared__ int64_t s_tmp[1024];
+    assign_volume(v, call.parity, flag_stride, blockIdx.x, gridDim.x, s_keys, s_tmp);
+}
+
+__global__ __launch_bounds__(1024) void k_assign_batch(BatchDev b) {
+    __shared__ int64_t s_keys[kSortTile];
+    __shared__ int64_t s_tmp[1024];
+    const int k = b.step_ctr ? *b.step_ctr : b.step;
+    const int sc = blockIdx.y;
+    if (!b.steps[k * b.n_scenes + sc].active) return;
+    const VolDev v = b.vols[sc];
+    assign_volume(v, batch_call(b, sc, k).parity, sort_flag_stride(v.capacity), blockIdx.x,
+                  gridDim.x, s_keys, s_tmp);
 }
 
 // ---- reference-semantics scatter update (volume.cpp:200-299) ---------------
@@ -469,57 +686,6 @@ __global__ void k_fill_table(HashEntry* t, uint32_t n) {
     if (i < n) t[i] = HashEntry{kEmptyKey, kIdxPending, 0};
 }
 
-__global__ void k_zero_words(uint32_t* p, size_t n) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * 256;
-    for (size_t k = i; k < n; k += stride) p[k] = 0u;
-}
-
-}  // namespace
-
-int32_t sort_flag_stride(uint32_t capacity) {
-    return (int32_t)((capacity + kSortTile - 1) / kSortTile / 2 + 1);
-}
-
-hipError_t launch_begin_call(const VolDev& v, hipStream_t s) {
-    k_begin_call<<<1, 1, 0, s>>>(v.ctr);
-    return hipGetLastError();
-}
-
-template <int MODE, bool TOUCH>
-static void launch_alloc_t(const VolDev& v, const PointSrc& src, const int32_t* coords, int set,
-                           hipStream_t s) {
-    if (coords) {
-        dim3 grid((src.n + kAllocThreads - 1) / kAllocThreads);
-        k_alloc<MODE, TOUCH, kSrcCoords><<<grid, kAllocThreads, 0, s>>>(v, src, coords, set);
-    } else if (src.depth) {
-        dim3 grid((src.W + 15) / 16, (src.H + 15) / 16);
-        k_alloc<MODE, TOUCH, kSrcDepth><<<grid, kAllocThreads, 0, s>>>(v, src, nullptr, set);
-    } else {
-        dim3 grid((src.n + kAllocThreads - 1) / kAllocThreads);
-        k_alloc<MODE, TOUCH, kSrcPoints><<<grid, kAllocThreads, 0, s>>>(v, src, nullptr, set);
-    }
-}
-
-hipError_t launch_alloc(const VolDev& v, const PointSrc& src, const AllocOpts& o,
-                        const int32_t* coords, hipStream_t s) {
-    int set = o.lds_set;
-    if (set < 0) set = 0;
-    if (set > 2048) set = 2048;
-    if (set & (set - 1)) return hipErrorInvalidValue;
-    switch (o.mode) {
-        case kAllocNormal:
-            if (o.track_touched) launch_alloc_t<kAllocNormal, true>(v, src, coords, set, s);
-            else launch_alloc_t<kAllocNormal, false>(v, src, coords, set, s);
-            break;
-        case kAllocCountCas: launch_alloc_t<kAllocCountCas, false>(v, src, coords, set, s); break;
-        case kAllocProbe: launch_alloc_t<kAllocProbe, false>(v, src, coords, set, s); break;
-        case kAllocNoCas: launch_alloc_t<kAllocNoCas, false>(v, src, coords, set, s); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 // Back-project every pixel (the same FP32 expressions as the depth allocate
 // path) into 3 floats per pixel, NaN for pixels that are skipped.
 __global__ void k_backproject(PointSrc src, float* out) {
@@ -540,15 +706,68 @@ __global__ void k_backproject(PointSrc src, float* out) {
     o[2] = src.R[6] * xc + src.R[7] * yc + src.R[8] * z + src.cam[2];
 }
 
-hipError_t launch_backproject(const PointSrc& src, float* out, hipStream_t s) {
-    dim3 grid((src.W + 15) / 16, (src.H + 15) / 16);
-    k_backproject<<<grid, 256, 0, s>>>(src, out);
+__global__ void k_step_advance(int32_t* step) { *step = *step + 1; }
+
+}  // namespace
+
+template <int MODE, bool TOUCH>
+static void launch_alloc_t(const VolDev& v, const PointSrc& src, const int32_t* coords, int set,
+                           CallInfo call, int grid_cap, hipStream_t s) {
+    int tiles;
+    if (src.depth && !coords) tiles = ((src.W + 15) / 16) * ((src.H + 15) / 16);
+    else tiles = (src.n + kAllocThreads - 1) / kAllocThreads;
+    const int grid = grid_cap > 0 ? min(tiles, grid_cap) : tiles;
+    if (coords)
+        k_alloc<MODE, TOUCH, kSrcCoords><<<grid, kAllocThreads, 0, s>>>(v, src, coords, set, call, tiles);
+    else if (src.depth)
+        k_alloc<MODE, TOUCH, kSrcDepth><<<grid, kAllocThreads, 0, s>>>(v, src, nullptr, set, call, tiles);
+    else
+        k_alloc<MODE, TOUCH, kSrcPoints><<<grid, kAllocThreads, 0, s>>>(v, src, nullptr, set, call, tiles);
+}
+
+static int clamp_set(int set) {
+    if (set < 0) set = 0;
+    if (set > kMaxSet) set = kMaxSet;
+    return set;
+}
+
+hipError_t launch_alloc(const VolDev& v, const PointSrc& src, const AllocOpts& o,
+                        const int32_t* coords, CallInfo call, hipStream_t s) {
+    const int set = clamp_set(o.lds_set);
+    if (set & (set - 1)) return hipErrorInvalidValue;
+    switch (o.mode) {
+        case kAllocNormal:
+            if (o.track_touched) launch_alloc_t<kAllocNormal, true>(v, src, coords, set, call, o.grid, s);
+            else launch_alloc_t<kAllocNormal, false>(v, src, coords, set, call, o.grid, s);
+            break;
+        case kAllocCountCas: launch_alloc_t<kAllocCountCas, false>(v, src, coords, set, call, o.grid, s); break;
+        case kAllocProbe: launch_alloc_t<kAllocProbe, false>(v, src, coords, set, call, o.grid, s); break;
+        case kAllocNoCas: launch_alloc_t<kAllocNoCas, false>(v, src, coords, set, call, o.grid, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_assign(const VolDev& v, hipStream_t s) {
+hipError_t launch_alloc_batch(const BatchDev& b, int32_t total_tiles, const AllocOpts& o,
+                              hipStream_t s) {
+    const int set = clamp_set(o.lds_set);
+    if (set & (set - 1)) return hipErrorInvalidValue;
+    const int grid = o.grid > 0 ? min(total_tiles, o.grid) : total_tiles;
+    k_alloc_batch<<<grid, kAllocThreads, 0, s>>>(b, set, total_tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_assign(const VolDev& v, CallInfo call, hipStream_t s) {
     const uint32_t tiles = (v.capacity + kSortTile - 1) / kSortTile;
-    k_assign<<<tiles, 1024, 0, s>>>(v, sort_flag_stride(v.capacity));
+    const uint32_t grid = tiles < (uint32_t)kAssignMaxGrid ? tiles : (uint32_t)kAssignMaxGrid;
+    k_assign<<<grid, 1024, 0, s>>>(v, call, sort_flag_stride(v.capacity));
+    return hipGetLastError();
+}
+
+hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStream_t s) {
+    const uint32_t tiles = (max_capacity + kSortTile - 1) / kSortTile;
+    const uint32_t gx = tiles < 64u ? tiles : 64u;
+    k_assign_batch<<<dim3(gx, b.n_scenes), 1024, 0, s>>>(b);
     return hipGetLastError();
 }
 
@@ -562,11 +781,14 @@ hipError_t launch_fill_table(HashEntry* table, uint32_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_zero_words(uint32_t* p, size_t n_words, hipStream_t s) {
-    size_t blocks = (n_words + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    if (blocks == 0) blocks = 1;
-    k_zero_words<<<(unsigned)blocks, 256, 0, s>>>(p, n_words);
+hipError_t launch_backproject(const PointSrc& src, float* out, hipStream_t s) {
+    dim3 grid((src.W + 15) / 16, (src.H + 15) / 16);
+    k_backproject<<<grid, 256, 0, s>>>(src, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_advance(int32_t* step_ctr, hipStream_t s) {
+    k_step_advance<<<1, 1, 0, s>>>(step_ctr);
     return hipGetLastError();
 }
 

@@ -68,6 +68,9 @@ struct OsnTsdfHipVolume {
     int32_t lds_set = 1024;
     int32_t update_threads = 256;
     int32_t update_grid = 0;
+    int32_t alloc_grid = 0;
+    int64_t calls = 0;  // allocate calls since create/reset: call c stamps c + 1, uses list set c & 1
+    bool in_batch = false;
     // extraction scratch, grown on demand
     int32_t* ex_vcount = nullptr;
     int32_t* ex_voff = nullptr;
@@ -82,7 +85,7 @@ struct OsnTsdfHipVolume {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double ms[kStages] = {0, 0, 0, 0};
-    int64_t calls[kStages] = {0, 0, 0, 0};
+    int64_t stage_calls[kStages] = {0, 0, 0, 0};
 
     ~OsnTsdfHipVolume();
     bool alloc();
@@ -153,6 +156,7 @@ bool OsnTsdfHipVolume::reset() {
         HIP_OK(hipMemsetAsync(d.vox, 0, n_vox * voxel_bytes(d.storage), stream));
     }
     HIP_OK(hipStreamSynchronize(stream));
+    calls = 0;
     ok = true;
     return true;
 }
@@ -265,6 +269,7 @@ OsnTsdfHipVolume* make_volume(const OsnTsdfHipConfig& c, bool reference_shape) {
     v->lds_set = c.lds_set < 0 ? 1024 : c.lds_set;
     v->update_threads = c.update_threads > 0 ? c.update_threads : 256;
     v->update_grid = c.update_grid;
+    v->alloc_grid = c.alloc_grid;
     if (!v->alloc()) {
         delete v;
         return nullptr;
@@ -310,19 +315,22 @@ ProjCam proj_cam(const float* depth, const OsnTsdfHipFrame* f) {
 
 int32_t run_allocate(OsnTsdfHipVolume* v, const PointSrc& src, int mode, bool touched,
                      const int32_t* coords = nullptr) {
+    if (v->in_batch) return 3;
     hipEvent_t e0, e1;
+    const CallInfo call{(int32_t)(v->calls + 1), (int32_t)(v->calls & 1)};
+    v->calls += 1;
     v->stage_begin(0, e0);
-    hipError_t err = launch_begin_call(v->d, v->stream);
     AllocOpts o;
     o.mode = mode;
     o.track_touched = touched;
     o.lds_set = v->lds_set;
-    if (err == hipSuccess) err = launch_alloc(v->d, src, o, coords, v->stream);
+    o.grid = v->alloc_grid;
+    hipError_t err = launch_alloc(v->d, src, o, coords, call, v->stream);
     v->stage_end(0, e0);
     if (err != hipSuccess) { v->fail("allocate launch", err); return 2; }
     if (mode == kAllocNoCas) return 0;
     v->stage_begin(1, e1);
-    err = launch_assign(v->d, v->stream);
+    err = launch_assign(v->d, call, v->stream);
     v->stage_end(1, e1);
     if (err != hipSuccess) { v->fail("assign launch", err); return 2; }
     return 0;
@@ -465,7 +473,9 @@ int32_t osn_tsdf_hip_update_depth(OsnTsdfHipVolume* v, const float* d_depth,
     if (v->d.storage == kStoreSums) return 3;
     hipEvent_t e;
     v->stage_begin(2, e);
-    const hipError_t err = launch_proj_update(v->d, proj_cam(d_depth, frame), v->update_grid,
+    if (v->calls == 0 || v->in_batch) { v->stage_end(2, e); return 3; }
+    const CallInfo last{(int32_t)v->calls, (int32_t)((v->calls - 1) & 1)};
+    const hipError_t err = launch_proj_update(v->d, proj_cam(d_depth, frame), last, v->update_grid,
                                               v->update_threads, v->stream);
     v->stage_end(2, e);
     if (err != hipSuccess) { v->fail("projective update launch", err); return 2; }
@@ -588,7 +598,10 @@ uint64_t osn_tsdf_hip_drop_count(const OsnTsdfHipVolume* v) {
 int32_t osn_tsdf_hip_touched_count(const OsnTsdfHipVolume* v) {
     if (!v || !v->ok) return 0;
     int32_t n = 0;
-    if (read_counter(v, offsetof(Counters, touched_count), &n)) return 0;
+    if (v->calls == 0) return 0;
+    const size_t off = offsetof(Counters, lists) + sizeof(ListCounters) * ((v->calls - 1) & 1) +
+                       offsetof(ListCounters, touched_count);
+    if (read_counter(v, off, &n)) return 0;
     return n;
 }
 
@@ -703,7 +716,7 @@ int32_t osn_tsdf_hip_read_timing(OsnTsdfHipVolume* v, double* ms4, int64_t* call
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
             v->ms[p.first] += ms;
-            v->calls[p.first] += 1;
+            v->stage_calls[p.first] += 1;
         }
         v->ev_pool.push_back(p.second.first);
         v->ev_pool.push_back(p.second.second);
@@ -711,13 +724,257 @@ int32_t osn_tsdf_hip_read_timing(OsnTsdfHipVolume* v, double* ms4, int64_t* call
     v->ev_pending.clear();
     for (int i = 0; i < kStages; ++i) {
         if (ms4) ms4[i] = v->ms[i];
-        if (calls4) calls4[i] = v->calls[i];
+        if (calls4) calls4[i] = v->stage_calls[i];
         v->ms[i] = 0.0;
-        v->calls[i] = 0;
+        v->stage_calls[i] = 0;
     }
     return 0;
 }
 
 void* osn_tsdf_hip_stream(const OsnTsdfHipVolume* v) { return v ? (void*)v->stream : nullptr; }
+
+}  // extern "C"
+
+// ---- batches ---------------------------------------------------------------
+
+struct OsnTsdfHipBatch {
+    std::vector<OsnTsdfHipVolume*> vols;
+    int32_t storage = kStoreF16;
+    hipStream_t stream = nullptr;
+    VolDev* d_vols = nullptr;
+    int32_t* d_call_base = nullptr;
+    int32_t* d_tile_prefix = nullptr;
+    int32_t* d_step = nullptr;
+    SceneStep* d_steps = nullptr;
+    int32_t n_steps = 0;
+    int32_t cursor = 0;
+    int32_t total_tiles = 0;
+    uint32_t max_capacity = 0;
+    std::vector<int64_t> base_calls;
+    int32_t alloc_grid = 0, update_grid = 0, update_threads = 256, lds_set = 1024;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool timing = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double ms[kStages] = {0, 0, 0, 0};
+    int64_t stage_calls[kStages] = {0, 0, 0, 0};
+
+    void drop_graph() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    BatchDev dev(int step, bool from_device) const {
+        BatchDev b{};
+        b.vols = d_vols;
+        b.steps = d_steps;
+        b.tile_prefix = d_tile_prefix;
+        b.step_ctr = from_device ? d_step : nullptr;
+        b.call_base = d_call_base;
+        b.step = step;
+        b.n_scenes = (int32_t)vols.size();
+        return b;
+    }
+    hipError_t enqueue_step(const BatchDev& b) {
+        AllocOpts o;
+        o.lds_set = lds_set;
+        o.grid = alloc_grid;
+        hipEvent_t e[6] = {};
+        auto mark = [&](int i) {
+            if (timing && hipEventCreate(&e[i]) == hipSuccess) hipEventRecord(e[i], stream);
+        };
+        mark(0);
+        hipError_t err = launch_alloc_batch(b, total_tiles, o, stream);
+        mark(1);
+        mark(2);
+        if (err == hipSuccess) err = launch_assign_batch(b, max_capacity, stream);
+        mark(3);
+        mark(4);
+        if (err == hipSuccess)
+            err = launch_proj_update_batch(b, storage, update_grid, update_threads, stream);
+        mark(5);
+        if (timing)
+            for (int i = 0; i < 3; ++i) ev_pending.push_back({i, {e[2 * i], e[2 * i + 1]}});
+        return err;
+    }
+    ~OsnTsdfHipBatch() {
+        if (stream) hipStreamSynchronize(stream);
+        drop_graph();
+        for (auto& p : ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
+        for (void* p : {(void*)d_vols, (void*)d_call_base, (void*)d_tile_prefix, (void*)d_step,
+                        (void*)d_steps})
+            if (p) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+        for (auto* v : vols) v->in_batch = false;
+    }
+};
+
+extern "C" {
+
+OsnTsdfHipBatch* osn_tsdf_hip_batch_create(OsnTsdfHipVolume* const* volumes, int32_t n) {
+    if (!volumes || n <= 0 || n > 64) return nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!volumes[i] || !volumes[i]->ok || volumes[i]->in_batch) return nullptr;
+        if (volumes[i]->d.storage == kStoreSums) return nullptr;
+        if (volumes[i]->d.storage != volumes[0]->d.storage) return nullptr;
+        for (int j = 0; j < i; ++j)
+            if (volumes[j] == volumes[i]) return nullptr;
+    }
+    auto* b = new (std::nothrow) OsnTsdfHipBatch();
+    if (!b) return nullptr;
+    b->storage = volumes[0]->d.storage;
+    std::vector<VolDev> hv;
+    for (int i = 0; i < n; ++i) {
+        OsnTsdfHipVolume* v = volumes[i];
+        hipStreamSynchronize(v->stream);
+        b->vols.push_back(v);
+        hv.push_back(v->d);
+        if (v->d.capacity > b->max_capacity) b->max_capacity = v->d.capacity;
+    }
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(&b->d_vols, sizeof(VolDev) * n) == hipSuccess &&
+              hipMalloc(&b->d_call_base, sizeof(int32_t) * n) == hipSuccess &&
+              hipMalloc(&b->d_tile_prefix, sizeof(int32_t) * (n + 1)) == hipSuccess &&
+              hipMalloc(&b->d_step, sizeof(int32_t)) == hipSuccess &&
+              hipMemcpy(b->d_vols, hv.data(), sizeof(VolDev) * n, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) { delete b; return nullptr; }
+    for (auto* v : b->vols) v->in_batch = true;
+    return b;
+}
+
+void osn_tsdf_hip_batch_destroy(OsnTsdfHipBatch* b) { delete b; }
+
+int32_t osn_tsdf_hip_batch_set_schedule(OsnTsdfHipBatch* b, int32_t n_steps,
+                                        const float* const* d_depths,
+                                        const OsnTsdfHipFrame* frames) {
+    if (!b || n_steps <= 0 || !d_depths || !frames) return 1;
+    const int n = (int)b->vols.size();
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return 2;
+    b->drop_graph();
+    std::vector<SceneStep> hs((size_t)n_steps * n);
+    std::vector<int32_t> tiles(n, 0);
+    for (int k = 0; k < n_steps; ++k)
+        for (int s = 0; s < n; ++s) {
+            const size_t i = (size_t)k * n + s;
+            SceneStep& st = hs[i];
+            st.active = 0;
+            if (!d_depths[i]) continue;
+            if (!valid_frame(d_depths[i], &frames[i])) return 1;
+            st.active = 1;
+            st.src = depth_src(d_depths[i], &frames[i]);
+            st.cam = proj_cam(d_depths[i], &frames[i]);
+            st.tiles = ((frames[i].width + 15) / 16) * ((frames[i].height + 15) / 16);
+            if (st.tiles > tiles[s]) tiles[s] = st.tiles;
+        }
+    std::vector<int32_t> prefix(n + 1, 0);
+    for (int s = 0; s < n; ++s) prefix[s + 1] = prefix[s] + tiles[s];
+    b->base_calls.assign(n, 0);
+    std::vector<int32_t> base(n);
+    for (int s = 0; s < n; ++s) {
+        b->base_calls[s] = b->vols[s]->calls;
+        base[s] = (int32_t)b->vols[s]->calls;
+    }
+    if (b->d_steps) hipFree(b->d_steps);
+    b->d_steps = nullptr;
+    if (hipMalloc(&b->d_steps, sizeof(SceneStep) * hs.size()) != hipSuccess) return 2;
+    if (hipMemcpy(b->d_steps, hs.data(), sizeof(SceneStep) * hs.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(b->d_tile_prefix, prefix.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) ||
+        hipMemcpy(b->d_call_base, base.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice))
+        return 2;
+    b->n_steps = n_steps;
+    b->cursor = 0;
+    b->total_tiles = prefix[n];
+    return 0;
+}
+
+int32_t osn_tsdf_hip_batch_run(OsnTsdfHipBatch* b, int32_t n_steps, int32_t use_graph) {
+    if (!b || n_steps <= 0 || !b->d_steps) return 1;
+    if (b->cursor + n_steps > b->n_steps) return 1;
+    if (!use_graph) {
+        for (int i = 0; i < n_steps; ++i)
+            if (b->enqueue_step(b->dev(b->cursor + i, false)) != hipSuccess) return 2;
+    } else {
+        const int32_t first = b->cursor;
+        if (hipMemcpyAsync(b->d_step, &first, sizeof(int32_t), hipMemcpyHostToDevice, b->stream) ||
+            hipStreamSynchronize(b->stream))
+            return 2;
+        if (!b->exec) {
+            const bool t = b->timing;
+            b->timing = false;
+            if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return 2;
+            hipError_t err = b->enqueue_step(b->dev(0, true));
+            if (err == hipSuccess) err = launch_step_advance(b->d_step, b->stream);
+            hipGraph_t g = nullptr;
+            const hipError_t end = hipStreamEndCapture(b->stream, &g);
+            b->timing = t;
+            if (err != hipSuccess || end != hipSuccess) { if (g) hipGraphDestroy(g); return 2; }
+            b->graph = g;
+            if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+                b->drop_graph();
+                return 2;
+            }
+        }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (b->timing) { hipEventCreate(&e0); hipEventRecord(e0, b->stream); }
+        for (int i = 0; i < n_steps; ++i)
+            if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) return 2;
+        if (b->timing) {
+            hipEventCreate(&e1);
+            hipEventRecord(e1, b->stream);
+            b->ev_pending.push_back({0, {e0, e1}});
+        }
+    }
+    b->cursor += n_steps;
+    for (size_t s = 0; s < b->vols.size(); ++s) b->vols[s]->calls = b->base_calls[s] + b->cursor;
+    return 0;
+}
+
+void osn_tsdf_hip_batch_synchronize(const OsnTsdfHipBatch* b) {
+    if (b && b->stream) hipStreamSynchronize(b->stream);
+}
+
+int32_t osn_tsdf_hip_batch_tune(OsnTsdfHipBatch* b, int32_t alloc_grid, int32_t update_grid,
+                                int32_t update_threads, int32_t lds_set) {
+    if (!b) return 1;
+    if (update_threads && update_threads != 128 && update_threads != 256 && update_threads != 512)
+        return 1;
+    if (lds_set > 1024 || (lds_set > 0 && (lds_set & (lds_set - 1)))) return 1;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return 2;
+    b->drop_graph();
+    if (alloc_grid >= 0) b->alloc_grid = alloc_grid;
+    if (update_grid >= 0) b->update_grid = update_grid;
+    if (update_threads) b->update_threads = update_threads;
+    if (lds_set >= 0) b->lds_set = lds_set;
+    return 0;
+}
+
+int32_t osn_tsdf_hip_batch_set_timing(OsnTsdfHipBatch* b, int32_t enable) {
+    if (!b) return 1;
+    b->timing = enable != 0;
+    return 0;
+}
+
+int32_t osn_tsdf_hip_batch_read_timing(OsnTsdfHipBatch* b, double* ms4, int64_t* calls4) {
+    if (!b) return 1;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return 2;
+    for (auto& p : b->ev_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            b->ms[p.first] += ms;
+            b->stage_calls[p.first] += 1;
+        }
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    b->ev_pending.clear();
+    for (int i = 0; i < kStages; ++i) {
+        if (ms4) ms4[i] = b->ms[i];
+        if (calls4) calls4[i] = b->stage_calls[i];
+        b->ms[i] = 0.0;
+        b->stage_calls[i] = 0;
+    }
+    return 0;
+}
 
 }  // extern "C"

@@ -47,22 +47,33 @@ enum Storage : int32_t {
     kStoreF64 = 3,   // projective FP64 oracle: {double tsdf_mean, double weight}
 };
 
-// Per-volume device counters, one allocation. The first two fields are what
-// the reference device view exposes as block_count (int32*) and drop_count
-// (uint64*).
+// Per-call list lengths. Allocate calls alternate between two sets (call
+// parity): a call appends to lists[p] and its first workgroup clears
+// lists[p ^ 1] for the next call, whose readers of lists[p ^ 1] (the previous
+// call's update) have finished by then because they ran earlier on the same
+// stream. Each set sits on its own 128-byte line so appends to one list do
+// not serialise against another's.
+struct alignas(128) ListCounters {
+    int32_t new_count;       // keys inserted by the call
+    int32_t pad0[31];
+    int32_t touched_count;   // distinct slots the call visited
+    int32_t pad1[31];
+};
+
+// Per-volume device counters, one allocation. block_count and drop_count are
+// what the reference device view exposes as int32* and uint64*.
 struct Counters {
     int32_t block_count;
-    int32_t stamp;            // incremented at the start of each allocate call
+    int32_t stamp_unused;
     unsigned long long drop_count;
-    int32_t new_count;        // keys inserted by the current call
-    int32_t touched_count;    // distinct slots the current call visited
     unsigned long long cas_attempts;
     unsigned long long lane_steps;   // sum over lanes of probe steps
     unsigned long long wave_steps;   // sum over wave batches of 64 * deepest lane
     unsigned long long lookups;      // distinct-key probes issued to the table
-    int32_t sort_levels_done;        // scratch for the assign kernel
     int32_t mesh_vertices;           // last extract's vertex count, -1 on overflow
+    int32_t pad;
     unsigned long long probe_hist[kProbeHistBins];
+    ListCounters lists[2];
 };
 
 // Everything a kernel needs to address one volume. Passed by value.

@@ -19,19 +19,6 @@ struct PointSrc {
     float radius_sq;       // band-walk far cull, 0 disables
 };
 
-enum AllocMode : int32_t {
-    kAllocNormal = 0,
-    kAllocCountCas = 1,  // tally every compare-exchange attempt
-    kAllocProbe = 2,     // probe-depth histogram and lane/wave step totals
-    kAllocNoCas = 3,     // probe and read only, never insert
-};
-
-struct AllocOpts {
-    int32_t mode = kAllocNormal;
-    bool track_touched = false;
-    int32_t lds_set = 1024;   // per-workgroup pre-hash entries (0 disables)
-};
-
 // Camera for the projective update: world -> camera rotation and the camera
 // centre, FP32, plus intrinsics and the depth image.
 struct ProjCam {
@@ -43,6 +30,56 @@ struct ProjCam {
     double Rcw64[9];   // FP64 pose, read only by the FP64 oracle kernel
     double c64[3];
 };
+
+// Which call this is: the stamp marks blocks touched by it, the parity picks
+// the list-counter set it appends to.
+struct CallInfo {
+    int32_t stamp;
+    int32_t parity;
+};
+
+enum AllocMode : int32_t {
+    kAllocNormal = 0,
+    kAllocCountCas = 1,  // tally every compare-exchange attempt
+    kAllocProbe = 2,     // probe-depth histogram and lane/wave step totals
+    kAllocNoCas = 3,     // probe and read only, never insert
+};
+
+struct AllocOpts {
+    int32_t mode = kAllocNormal;
+    bool track_touched = false;
+    int32_t lds_set = 1024;   // per-workgroup pre-hash entries (0 disables)
+    int32_t grid = 0;         // workgroups; 0 = one per tile up to a cap
+};
+
+// One scene of a batched step: its volume, its frame (allocation source and
+// projection camera) and whether it has a frame this step.
+struct SceneStep {
+    PointSrc src;
+    ProjCam cam;
+    int32_t active;
+    int32_t tiles;      // 16x16 pixel tiles of this scene's image
+};
+
+// A batched step over many scenes, read from device memory so that a captured
+// graph can replay it: scene s of step k is steps[k * n_scenes + s], with k
+// read from *step_ctr when step_ctr is set, else `step`.
+struct BatchDev {
+    const VolDev* vols;
+    const SceneStep* steps;
+    const int32_t* tile_prefix;  // n_scenes + 1, exclusive prefix of tiles
+    const int32_t* step_ctr;
+    const int32_t* call_base;    // per scene: calls the volume had made before step 0
+    int32_t step;
+    int32_t n_scenes;
+};
+
+// Call number c of a volume (0-based) stamps its blocks with c + 1 and
+// appends to list set c & 1; step k of a batch is call call_base[s] + k.
+__device__ inline CallInfo batch_call(const BatchDev& b, int sc, int k) {
+    const int32_t c = b.call_base[sc] + k;
+    return CallInfo{c + 1, c & 1};
+}
 
 struct ExtractArgs {
     float min_weight;
@@ -60,19 +97,26 @@ struct ExtractArgs {
 
 constexpr int kSortTile = 4096;
 constexpr int kMaxSortLevels = 24;
+constexpr int kAssignMaxGrid = 256;
 
-hipError_t launch_begin_call(const VolDev& v, hipStream_t s);
 hipError_t launch_alloc(const VolDev& v, const PointSrc& src, const AllocOpts& o,
-                        const int32_t* coords, hipStream_t s);
+                        const int32_t* coords, CallInfo call, hipStream_t s);
+hipError_t launch_alloc_batch(const BatchDev& b, int32_t total_tiles, const AllocOpts& o,
+                              hipStream_t s);
 hipError_t launch_backproject(const PointSrc& src, float* out, hipStream_t s);
-hipError_t launch_assign(const VolDev& v, hipStream_t s);
+hipError_t launch_assign(const VolDev& v, CallInfo call, hipStream_t s);
+hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStream_t s);
 hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, hipStream_t s);
-hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, int32_t grid,
+hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call, int32_t grid,
                               int32_t block_threads, hipStream_t s);
+hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t grid,
+                                    int32_t block_threads, hipStream_t s);
+hipError_t launch_step_advance(int32_t* step_ctr, hipStream_t s);
 hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipStream_t s);
 hipError_t launch_fill_table(HashEntry* table, uint32_t n, hipStream_t s);
-hipError_t launch_zero_words(uint32_t* p, size_t n_words, hipStream_t s);
 
-int32_t sort_flag_stride(uint32_t capacity);
+__host__ __device__ inline int32_t sort_flag_stride(uint32_t capacity) {
+    return (int32_t)((capacity + kSortTile - 1) / kSortTile / 2 + 1);
+}
 
 }  // namespace osn_hip

@@ -5,7 +5,8 @@
 // every voxel of every touched block once, projects its centre into the depth
 // image and folds the observation into a running weighted average:
 //
-//   p   = R_cw * ((v + 0.5) * voxel - c)          camera coordinates (FP32)
+//   p   = R_cw * ((v + 0.5) * voxel - c)          camera coordinates (FP32),
+//         evaluated as row = (d0 + ly*ey) + lz*ez, p = row + lx*ex
 //   iz  = 1 / p.z                                  (skip unless p.z > 0)
 //   pix = floor(fx * (p.x * iz) + cx + 0.5), floor(fy * (p.y * iz) + cy + 0.5)
 //   d   = depth[pix]                               (skip unless inside, d > 0)
@@ -13,10 +14,11 @@
 //   x   = min(sdf * (1 / trunc), 1)
 //   t  <- (t * w + x) / (w + 1),  w <- min(w + 1, w_max)
 //
-// One workgroup slice per 8^3 block, each thread owning 16 contiguous bytes of
-// the block's voxel record (4 voxels of {half, u16}, 2 of {float, float}, 1 of
-// {double, double}), so every load and store is a coalesced 16-byte access and
-// each voxel is owned by exactly one thread: no atomics, deterministic.
+// One wave per 8^3 block: each load of a lane is 16 contiguous bytes of the
+// block's voxel record (4 voxels of {half, u16}, 2 of {float, float}, 1 of
+// {double, double}), consecutive lanes read consecutive 16-byte pieces, and
+// each voxel is owned by exactly one lane: no atomics, deterministic. The
+// voxels of one load share (ly, lz), so their row term is computed once.
 // The discrete choices (pixel, gates) are made by the same FP32 expression
 // sequence in every storage mode, including the FP64 oracle kernel, which
 // then evaluates the signed distance and the average in FP64 from the FP64
@@ -30,13 +32,54 @@
 namespace osn_hip {
 namespace {
 
+// Everything the update reads for one scene, gathered once per launch (kernel
+// arguments in the single-volume kernel, LDS in the batched one) so that the
+// per-voxel loop never re-reads descriptors from global memory.
+struct UpdParams {
+    const HashEntry* table;
+    const int32_t* block_coord;
+    void* vox;
+    const uint32_t* touched;
+    const float* depth;
+    float Rcw[9];
+    float c[3];
+    float fx, fy, cx, cy;
+    int32_t W, H;
+    float voxel, trunc, inv_trunc, wmax;
+    int32_t n;          // touched blocks
+    int32_t pad;
+    double Rcw64[9];    // FP64 oracle kernel only
+    double c64[3];
+};
+
+__device__ inline UpdParams make_params(const VolDev& v, const ProjCam& k, int parity) {
+    UpdParams p;
+    p.table = v.table;
+    p.block_coord = v.block_coord;
+    p.vox = v.vox;
+    p.touched = v.touched;
+    p.depth = k.depth;
+    for (int i = 0; i < 9; ++i) { p.Rcw[i] = k.Rcw[i]; p.Rcw64[i] = k.Rcw64[i]; }
+    for (int i = 0; i < 3; ++i) { p.c[i] = k.c[i]; p.c64[i] = k.c64[i]; }
+    p.fx = k.fx; p.fy = k.fy; p.cx = k.cx; p.cy = k.cy;
+    p.W = k.W; p.H = k.H;
+    p.voxel = v.voxel;
+    p.trunc = v.trunc;
+    p.inv_trunc = 1.0f / v.trunc;
+    p.wmax = (float)v.w_max;
+    p.n = v.ctr->lists[parity].touched_count;
+    p.pad = 0;
+    return p;
+}
+
 struct BlockFrame {
     float d0[3];     // camera coords of the block's first voxel centre
     float e[3][3];   // e[j] = voxel * R_cw[:, j]
 };
 
-__device__ inline void block_frame(const ProjCam& k, float voxel, int32_t bx, int32_t by,
-                                   int32_t bz, BlockFrame& f) {
+__device__ inline void block_frame(const UpdParams& k, int32_t bx, int32_t by, int32_t bz,
+                                   BlockFrame& f) {
+    const float voxel = k.voxel;
     const float h = 0.5f * voxel;
     const float ox = (float)(bx * 8) * voxel + h - k.c[0];
     const float oy = (float)(by * 8) * voxel + h - k.c[1];
@@ -49,24 +92,20 @@ __device__ inline void block_frame(const ProjCam& k, float voxel, int32_t bx, in
     }
 }
 
-// FP32 decision for one voxel. Returns false when the voxel gets no
+// FP32 decision for the voxel at camera point p. Returns false when it gets no
 // observation; otherwise the pixel's depth and the FP32 sdf.
-__device__ inline bool observe(const ProjCam& k, const BlockFrame& f, float trunc, int lx, int ly,
-                               int lz, float& sdf_out, float& depth_out) {
-    const float fl0 = (float)lx, fl1 = (float)ly, fl2 = (float)lz;
-    const float p0 = f.d0[0] + fl0 * f.e[0][0] + fl1 * f.e[1][0] + fl2 * f.e[2][0];
-    const float p1 = f.d0[1] + fl0 * f.e[0][1] + fl1 * f.e[1][1] + fl2 * f.e[2][1];
-    const float p2 = f.d0[2] + fl0 * f.e[0][2] + fl1 * f.e[1][2] + fl2 * f.e[2][2];
+__device__ inline bool observe(const UpdParams& k, float p0, float p1, float p2, float& sdf_out,
+                               float& depth_out) {
     if (!(p2 > 0.0f)) return false;
     const float iz = 1.0f / p2;
     const float uf = k.fx * (p0 * iz) + k.cx + 0.5f;
     const float vf = k.fy * (p1 * iz) + k.cy + 0.5f;
     if (!(uf >= 0.0f) || !(vf >= 0.0f) || !(uf < (float)k.W) || !(vf < (float)k.H)) return false;
     const int u = (int)uf, w = (int)vf;  // uf, vf >= 0: truncation is floor
-    const float d = k.depth[(size_t)w * k.W + u];
+    const float d = k.depth[w * k.W + u];
     if (!(d > 0.0f)) return false;
     const float sdf = d - p2;
-    if (sdf < -trunc) return false;
+    if (sdf < -k.trunc) return false;
     sdf_out = sdf;
     depth_out = d;
     return true;
@@ -81,118 +120,185 @@ __device__ inline float avg_step(float t, float w, float x) {
 template <int STORE>
 struct Layout;
 template <>
-struct Layout<kStoreF16> { static constexpr int kVpt = 4; };
+struct Layout<kStoreF16> { static constexpr int kVpl = 4; };  // voxels per 16-byte load
 template <>
-struct Layout<kStoreF32> { static constexpr int kVpt = 2; };
+struct Layout<kStoreF32> { static constexpr int kVpl = 2; };
 template <>
-struct Layout<kStoreF64> { static constexpr int kVpt = 1; };
+struct Layout<kStoreF64> { static constexpr int kVpl = 1; };
 
+// One wave updates the whole block stored at table slot `slot` (uniform over
+// the wave). Load i of lane l covers voxels [i*64*V + l*V, +V).
 template <int STORE>
-__global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam k) {
-    constexpr int kVpt = Layout<STORE>::kVpt;
-    constexpr int kTpb = kBlockVoxels / kVpt;  // threads per block record
-    const int per_wg = blockDim.x / kTpb;        // block records per workgroup pass
-    const int sub = threadIdx.x / kTpb;
-    const int t = threadIdx.x % kTpb;
-    const int n = v.ctr->touched_count;
-    const float inv_trunc = 1.0f / v.trunc;
-    const float wmax = (float)v.w_max;
+__device__ inline void update_block(const UpdParams& k, uint32_t slot, int lane) {
+    constexpr int V = Layout<STORE>::kVpl;
+    constexpr int L = kBlockVoxels / (64 * V);
+    const int32_t bi = k.table[slot].block_idx;
+    if (bi < 0) return;
+    const int32_t bx = k.block_coord[bi * 3 + 0];
+    const int32_t by = k.block_coord[bi * 3 + 1];
+    const int32_t bz = k.block_coord[bi * 3 + 2];
+    BlockFrame f;
+    block_frame(k, bx, by, bz, f);
 
-    for (int item = blockIdx.x * per_wg + sub; item < n; item += gridDim.x * per_wg) {
-        const uint32_t slot = v.touched[item];
-        const int32_t bi = v.table[slot].block_idx;
-        if (bi < 0) continue;
-        const int32_t bx = v.block_coord[bi * 3 + 0];
-        const int32_t by = v.block_coord[bi * 3 + 1];
-        const int32_t bz = v.block_coord[bi * 3 + 2];
-        BlockFrame f;
-        block_frame(k, v.voxel, bx, by, bz, f);
-        const int vi0 = t * kVpt;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const int vi0 = i * 64 * V + lane * V;
+        const int lx0 = vi0 & 7, ly = (vi0 >> 3) & 7, lz = vi0 >> 6;
+        const float fly = (float)ly, flz = (float)lz;
+        float row[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) row[c] = (f.d0[c] + fly * f.e[1][c]) + flz * f.e[2][c];
 
         if constexpr (STORE == kStoreF16) {
-            uint4* rec = reinterpret_cast<uint4*>(static_cast<uint32_t*>(v.vox) +
-                                                  (size_t)bi * kBlockVoxels) + t;
-            uint4 q = *rec;
+            uint4* rec = reinterpret_cast<uint4*>(static_cast<uint32_t*>(k.vox) +
+                                                  (size_t)bi * kBlockVoxels + vi0);
+            const uint4 q = *rec;
             uint32_t word[4] = {q.x, q.y, q.z, q.w};
             bool dirty = false;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int vi = vi0 + j;
+                const float flx = (float)(lx0 + j);
                 float sdf, d;
-                if (!observe(k, f, v.trunc, vi & 7, (vi >> 3) & 7, vi >> 6, sdf, d)) continue;
-                const float x = fminf(sdf * inv_trunc, 1.0f);
+                if (!observe(k, row[0] + flx * f.e[0][0], row[1] + flx * f.e[0][1],
+                             row[2] + flx * f.e[0][2], sdf, d))
+                    continue;
+                const float x = fminf(sdf * k.inv_trunc, 1.0f);
                 const float tt = __half2float(__ushort_as_half((unsigned short)(word[j] & 0xffffu)));
                 const float w = (float)(word[j] >> 16);
                 const float tn = avg_step(tt, w, x);
-                const float wn = fminf(w + 1.0f, wmax);
-                word[j] = (uint32_t)__half_as_ushort(__float2half_rn(tn)) |
-                          ((uint32_t)wn << 16);
+                const float wn = fminf(w + 1.0f, k.wmax);
+                word[j] = (uint32_t)__half_as_ushort(__float2half_rn(tn)) | ((uint32_t)wn << 16);
                 dirty = true;
             }
             if (dirty) *rec = make_uint4(word[0], word[1], word[2], word[3]);
         } else if constexpr (STORE == kStoreF32) {
-            float4* rec = reinterpret_cast<float4*>(static_cast<float*>(v.vox) +
-                                                    (size_t)bi * kBlockVoxels * 2) + t;
-            float4 q = *rec;
+            float4* rec = reinterpret_cast<float4*>(static_cast<float*>(k.vox) +
+                                                    ((size_t)bi * kBlockVoxels + vi0) * 2);
+            const float4 q = *rec;
             float tv[2] = {q.x, q.z}, wv[2] = {q.y, q.w};
             bool dirty = false;
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                const int vi = vi0 + j;
+                const float flx = (float)(lx0 + j);
                 float sdf, d;
-                if (!observe(k, f, v.trunc, vi & 7, (vi >> 3) & 7, vi >> 6, sdf, d)) continue;
-                const float x = fminf(sdf * inv_trunc, 1.0f);
+                if (!observe(k, row[0] + flx * f.e[0][0], row[1] + flx * f.e[0][1],
+                             row[2] + flx * f.e[0][2], sdf, d))
+                    continue;
+                const float x = fminf(sdf * k.inv_trunc, 1.0f);
                 tv[j] = avg_step(tv[j], wv[j], x);
-                wv[j] = fminf(wv[j] + 1.0f, wmax);
+                wv[j] = fminf(wv[j] + 1.0f, k.wmax);
                 dirty = true;
             }
             if (dirty) *rec = make_float4(tv[0], wv[0], tv[1], wv[1]);
         } else {
-            double2* rec = reinterpret_cast<double2*>(static_cast<double*>(v.vox) +
-                                                      (size_t)bi * kBlockVoxels * 2) + t;
-            const int vi = vi0;
-            const int lx = vi & 7, ly = (vi >> 3) & 7, lz = vi >> 6;
+            double2* rec = reinterpret_cast<double2*>(static_cast<double*>(k.vox) +
+                                                      ((size_t)bi * kBlockVoxels + vi0) * 2);
+            const float flx = (float)lx0;
             float sdf, d;
-            if (observe(k, f, v.trunc, lx, ly, lz, sdf, d)) {
+            if (observe(k, row[0] + flx * f.e[0][0], row[1] + flx * f.e[0][1],
+                        row[2] + flx * f.e[0][2], sdf, d)) {
                 double2 q = *rec;
-                const double vs = (double)v.voxel;
-                const double wx = ((double)(bx * 8 + lx) + 0.5) * vs - k.c64[0];
+                const double vs = (double)k.voxel;
+                const double wx = ((double)(bx * 8 + lx0) + 0.5) * vs - k.c64[0];
                 const double wy = ((double)(by * 8 + ly) + 0.5) * vs - k.c64[1];
                 const double wz = ((double)(bz * 8 + lz) + 0.5) * vs - k.c64[2];
                 const double z = k.Rcw64[6] * wx + k.Rcw64[7] * wy + k.Rcw64[8] * wz;
-                double x = ((double)d - z) / (double)v.trunc;
+                double x = ((double)d - z) / (double)k.trunc;
                 x = x > 1.0 ? 1.0 : (x < -1.0 ? -1.0 : x);
                 const double w = q.y;
                 q.x = (q.x * w + x) / (w + 1.0);
-                q.y = (w + 1.0 < (double)v.w_max) ? w + 1.0 : (double)v.w_max;
+                q.y = (w + 1.0 < (double)k.wmax) ? w + 1.0 : (double)k.wmax;
                 *rec = q;
             }
         }
     }
 }
 
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+template <int STORE>
+__global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, CallInfo call) {
+    const int waves = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const UpdParams k = make_params(v, cam, call.parity);
+    for (int item = blockIdx.x * waves + wave_id(); item < k.n; item += gridDim.x * waves)
+        update_block<STORE>(k, k.touched[item], lane);
+}
+
+// All scenes of a batched step in one launch: the work list is the
+// concatenation of every active scene's touched blocks. Each workgroup gathers
+// the scenes' parameters and the prefix of their touched counts into LDS. The
+// list is cut into one contiguous chunk per XCD (workgroups are dealt to the
+// 8 XCDs round-robin by id), so each XCD's L2 serves the depth images of only
+// the few scenes in its chunk.
+constexpr int kMaxBatchScenes = 64;
+constexpr int kXcds = 8;
+
+template <int STORE>
+__global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
+    __shared__ UpdParams s_p[kMaxBatchScenes];
+    __shared__ int s_prefix[kMaxBatchScenes + 1];
+    const int waves = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int k = b.step_ctr ? *b.step_ctr : b.step;
+    const int ns = b.n_scenes;
+    if ((int)threadIdx.x < ns) {
+        const int sc = threadIdx.x;
+        const SceneStep& st = b.steps[k * ns + sc];
+        s_p[sc] = make_params(b.vols[sc], st.cam, batch_call(b, sc, k).parity);
+        if (!st.active) s_p[sc].n = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int sc = 0; sc < ns; ++sc) { s_prefix[sc] = acc; acc += s_p[sc].n; }
+        s_prefix[ns] = acc;
+    }
+    __syncthreads();
+    const int n = s_prefix[ns];
+    const int xcd = blockIdx.x % kXcds;
+    const int per_xcd = gridDim.x / kXcds;  // the launcher keeps the grid a multiple of 8
+    const int lo = (int)((long long)n * xcd / kXcds);
+    const int hi = (int)((long long)n * (xcd + 1) / kXcds);
+    int sc = 0;
+    for (int item = lo + (int)(blockIdx.x / kXcds) * waves + wave_id(); item < hi;
+         item += per_xcd * waves) {
+        while (s_prefix[sc + 1] <= item) ++sc;  // items ascend, so the scene only moves forward
+        const UpdParams& p = s_p[sc];
+        update_block<STORE>(p, p.touched[item - s_prefix[sc]], lane);
+    }
+}
+
 }  // namespace
 
-hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, int32_t grid,
+hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call, int32_t grid,
                               int32_t block_threads, hipStream_t s) {
     if (grid <= 0) grid = 2048;
-    if (block_threads != 256 && block_threads != 128 && block_threads != 512)
+    if (block_threads != 256 && block_threads != 128 && block_threads != 512 &&
+        block_threads != 64)
         return hipErrorInvalidValue;
     switch (v.storage) {
-        case kStoreF16:
-            if (block_threads < 128) return hipErrorInvalidValue;
-            k_proj_update<kStoreF16><<<grid, block_threads, 0, s>>>(v, cam);
-            break;
-        case kStoreF32:
-            if (block_threads < 256) return hipErrorInvalidValue;
-            k_proj_update<kStoreF32><<<grid, block_threads, 0, s>>>(v, cam);
-            break;
-        case kStoreF64:
-            if (block_threads < 512) block_threads = 512;
-            k_proj_update<kStoreF64><<<grid, block_threads, 0, s>>>(v, cam);
-            break;
-        default:
-            return hipErrorInvalidValue;
+        case kStoreF16: k_proj_update<kStoreF16><<<grid, block_threads, 0, s>>>(v, cam, call); break;
+        case kStoreF32: k_proj_update<kStoreF32><<<grid, block_threads, 0, s>>>(v, cam, call); break;
+        case kStoreF64: k_proj_update<kStoreF64><<<grid, block_threads, 0, s>>>(v, cam, call); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t grid,
+                                    int32_t block_threads, hipStream_t s) {
+    if (b.n_scenes > kMaxBatchScenes) return hipErrorInvalidValue;
+    if (grid <= 0) grid = 4096;
+    grid = (grid + kXcds - 1) / kXcds * kXcds;
+    if (block_threads != 256 && block_threads != 128 && block_threads != 512)
+        return hipErrorInvalidValue;
+    if (block_threads < b.n_scenes) block_threads = 512;  // one thread per scene gathers
+    switch (storage) {
+        case kStoreF16: k_proj_update_batch<kStoreF16><<<grid, block_threads, 0, s>>>(b); break;
+        case kStoreF32: k_proj_update_batch<kStoreF32><<<grid, block_threads, 0, s>>>(b); break;
+        case kStoreF64: k_proj_update_batch<kStoreF64><<<grid, block_threads, 0, s>>>(b); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -42,7 +42,8 @@ class Config(C.Structure):
                 ("trunc_m", C.c_float), ("weight_cap", C.c_float), ("storage", C.c_int32),
                 ("table_slots", C.c_uint32), ("hash_kind", C.c_int32),
                 ("probe_limit", C.c_uint32), ("lds_set", C.c_int32),
-                ("update_threads", C.c_int32), ("update_grid", C.c_int32)]
+                ("update_threads", C.c_int32), ("update_grid", C.c_int32),
+                ("alloc_grid", C.c_int32)]
 
 
 class Frame(C.Structure):
@@ -115,6 +116,14 @@ def lib():
         "osn_tsdf_hip_set_timing": (_I, [_VP, _I]),
         "osn_tsdf_hip_read_timing": (_I, [_VP, _VP, _VP]),
         "osn_tsdf_hip_stream": (_VP, [_VP]),
+        "osn_tsdf_hip_batch_create": (_VP, [_VP, _I]),
+        "osn_tsdf_hip_batch_destroy": (None, [_VP]),
+        "osn_tsdf_hip_batch_set_schedule": (_I, [_VP, _I, _VP, _VP]),
+        "osn_tsdf_hip_batch_run": (_I, [_VP, _I, _I]),
+        "osn_tsdf_hip_batch_synchronize": (None, [_VP]),
+        "osn_tsdf_hip_batch_tune": (_I, [_VP, _I, _I, _I, _I]),
+        "osn_tsdf_hip_batch_set_timing": (_I, [_VP, _I]),
+        "osn_tsdf_hip_batch_read_timing": (_I, [_VP, _VP, _VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -152,14 +161,14 @@ class HipVolume:
     def __init__(self, voxel: float, pool: int, trunc: float = -1.0, weight_cap: float = 32.0, *,
                  storage: int | None = None, table_slots: int = 0, hash_kind: int = HASH_REFERENCE,
                  probe_limit: int = 0, lds_set: int = -1, update_threads: int = 0,
-                 update_grid: int = 0):
+                 update_grid: int = 0, alloc_grid: int = 0):
         self._L = lib()
         if storage is None:
             self.h = self._L.osn_tsdf_hip_create(voxel, pool, trunc, weight_cap)
             self.storage = STORE_SUMS
         else:
             cfg = Config(voxel, pool, trunc, weight_cap, storage, table_slots, hash_kind,
-                         probe_limit, lds_set, update_threads, update_grid)
+                         probe_limit, lds_set, update_threads, update_grid, alloc_grid)
             self.h = self._L.osn_tsdf_hip_create_ex(C.byref(cfg))
             self.storage = storage
         if not self.h:
@@ -315,3 +324,57 @@ class HipVolume:
 
 def backproject(d_depth, frame: Frame, d_out) -> int:
     return lib().osn_tsdf_hip_backproject(_dptr(d_depth), C.byref(frame), _dptr(d_out))
+
+
+class Batch:
+    """Many projective volumes integrated together (one launch per stage per
+    step, optionally replayed from a captured hipGraph)."""
+
+    def __init__(self, volumes):
+        self._L = lib()
+        self.volumes = list(volumes)
+        arr = (C.c_void_p * len(self.volumes))(*[v.h for v in self.volumes])
+        self.h = self._L.osn_tsdf_hip_batch_create(arr, len(self.volumes))
+        if not self.h:
+            raise RuntimeError("osn_tsdf_hip_batch_create failed")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.osn_tsdf_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_schedule(self, depths, frames) -> int:
+        """depths[k][s]: device tensor or None; frames[k][s]: Frame."""
+        n_steps = len(depths)
+        n = len(self.volumes)
+        ptrs = (C.c_void_p * (n_steps * n))()
+        frs = (Frame * (n_steps * n))()
+        for k in range(n_steps):
+            for s in range(n):
+                d = depths[k][s]
+                ptrs[k * n + s] = None if d is None else _dptr(d)
+                if d is not None:
+                    frs[k * n + s] = frames[k][s]
+        return self._L.osn_tsdf_hip_batch_set_schedule(self.h, n_steps, ptrs, frs)
+
+    def run(self, n_steps: int, use_graph: bool = False) -> int:
+        return self._L.osn_tsdf_hip_batch_run(self.h, n_steps, 1 if use_graph else 0)
+
+    def synchronize(self):
+        self._L.osn_tsdf_hip_batch_synchronize(self.h)
+
+    def tune(self, alloc_grid=-1, update_grid=-1, update_threads=0, lds_set=-1) -> int:
+        return self._L.osn_tsdf_hip_batch_tune(self.h, alloc_grid, update_grid, update_threads,
+                                               lds_set)
+
+    def set_timing(self, on: bool):
+        self._L.osn_tsdf_hip_batch_set_timing(self.h, 1 if on else 0)
+
+    def read_timing(self):
+        ms = np.zeros(4, np.float64)
+        calls = np.zeros(4, np.int64)
+        self._L.osn_tsdf_hip_batch_read_timing(self.h, _hptr(ms), _hptr(calls))
+        return ms, calls
