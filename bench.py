@@ -58,8 +58,19 @@ def render_frames(H, S, cfg, seed, n):
     return frames, depth
 
 
-def make_volume(H, cfg, storage):
-    return H.HipVolume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 32.0, storage=storage)
+KNOBS = {}  # launch shape chosen by the autotuner (empty = library defaults)
+
+
+def make_volume(H, cfg, storage, hash_kind=None, pool=None):
+    """Table cfg["table"] slots; pool table/2 blocks unless given (the tuner's
+    short trials use a smaller pool over the same table)."""
+    hk = KNOBS.get("hash_kind", 0) if hash_kind is None else hash_kind
+    return H.HipVolume(cfg["voxel"], pool or cfg["table"] // 2, cfg["trunc"], 32.0,
+                       storage=storage, table_slots=cfg["table"],
+                       hash_kind=hk, lds_set=KNOBS.get("lds_set", -1),
+                       alloc_grid=KNOBS.get("alloc_grid", 0),
+                       update_threads=KNOBS.get("update_threads", 0),
+                       update_grid=KNOBS.get("update_grid", 0))
 
 
 def run_plain(H, cfg, storage, scenes, warmup, steps):
@@ -88,6 +99,9 @@ def run_plain(H, cfg, storage, scenes, warmup, steps):
 def make_batch(H, cfg, storage, scenes):
     vols = [make_volume(H, cfg, storage) for _ in scenes]
     b = H.Batch(vols)
+    if KNOBS and b.tune(KNOBS["alloc_grid"], KNOBS["update_grid"], KNOBS["update_threads"],
+                        KNOBS["lds_set"]):
+        raise RuntimeError("batch tune failed")
     n = scenes[0][1].shape[0]
     depths = [[sc[1][k] for sc in scenes] for k in range(n)]
     frames = [[sc[0][k] for sc in scenes] for k in range(n)]
@@ -216,6 +230,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="library default launch shape")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the batched fp16 run and one 256 MiB device copy (the copy "
                          "calibrates HBM byte counters), for rocprofv3 passes")
@@ -235,6 +250,23 @@ def main():
     log(f"rendering {args.scenes} scenes x {n} frames {cfg['width']}x{cfg['height']}")
     scenes = [render_frames(H, S, cfg, args.seed + i, n) for i in range(args.scenes)]
     frames_timed = args.steps * args.scenes
+
+    tune_log = None
+    if not args.no_tune:
+        import autotune
+        log("autotuning the launch shape on the first frames")
+        n_t = min(n, 30)
+        depths = [[sc[1][k] for sc in scenes] for k in range(n_t)]
+        frs = [[sc[0][k] for sc in scenes] for k in range(n_t)]
+        best, tlog = autotune.tune(
+            H, lambda hk: [make_volume(H, cfg, H.STORE_F16, hash_kind=hk, pool=1 << 17)
+                           for _ in scenes],
+            depths, frs, warmup=min(10, n_t // 3), steps=n_t - min(10, n_t // 3),
+            log_path=os.path.join(ROOT, "gpurun_out", "autotune.json")
+            if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else None)
+        KNOBS.update(best)
+        tune_log = {"chosen": best, "trials": len(tlog["trials"]), "seconds": tlog["seconds"]}
+        log(f"chosen {best}")
 
     if args.profile_only:
         src = torch.empty(1 << 26, dtype=torch.float32, device="cuda")
@@ -316,6 +348,7 @@ def main():
                      "avg_launch_ms": round(float(upd_ms16), 5),
                      "bytes_per_launch": round(float(bytes16.mean()), 0)},
         "cpu_baseline": cpu,
+        "autotune": tune_log,
     }
     print(json.dumps(out), flush=True)
 

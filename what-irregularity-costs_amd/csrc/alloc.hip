@@ -27,7 +27,7 @@ namespace {
 
 constexpr int kAllocThreads = 256;  // 4 waves; 16x16 pixels in depth mode
 constexpr int kKeysPerLane = 8;     // distinct keys a lane buffers before a flush
-constexpr int kWgList = 1024;       // LDS list capacity per workgroup
+constexpr int kWgList = 512;        // LDS list capacity per workgroup
 constexpr int kMaxSet = 1024;       // LDS pre-hash capacity
 
 enum SrcKind : int { kSrcPoints = 0, kSrcDepth = 1, kSrcCoords = 2 };
@@ -51,7 +51,7 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long x) {
 struct WgShared {
     int64_t set[kMaxSet];
     int64_t new_keys[kWgList];
-    uint32_t touched[kWgList];
+    TouchEntry touched[kWgList];
     int32_t n_new, n_touched;
     int32_t base_new, base_touched;
     unsigned long long drops, cas, lookups, lane_steps, wave_steps;
@@ -153,6 +153,7 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
     bool inserted = false, found = false;
     uint32_t slot = 0;
     uint32_t steps = 0;
+    int32_t idx = kIdxPending;
     unsigned long long cas = 0, drops = 0;
     if (go) {
         slot = home_slot(v, key);
@@ -174,8 +175,8 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
         }
         if (MODE != kAllocNoCas && !found && !inserted) drops += 1;  // probe bound hit
         if (found) {
-            const int32_t idx = __hip_atomic_load(&v.table[slot].block_idx, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
+            idx = __hip_atomic_load(&v.table[slot].block_idx, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
             if (idx == kIdxDropped) drops += 1;
         }
     }
@@ -209,7 +210,8 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
         touch = (old != c.stamp);
     }
     if (TOUCH)
-        wave_append<uint32_t>(touch, slot, &s.n_touched, s.touched, &c.lc->touched_count, v.touched);
+        wave_append<TouchEntry>(touch, TouchEntry{key, slot, idx}, &s.n_touched, s.touched,
+                                &c.lc->touched_count, v.touched);
     if (MODE != kAllocNoCas)
         wave_append<int64_t>(inserted, key, &s.n_new, s.new_keys, &c.lc->new_count, v.new_keys);
 }
@@ -342,6 +344,16 @@ __device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t*
                             fabsf(src.cam[1]) + fabsf(src.cam[2]) + dist +
                             (float)v.steps * v.voxel);
     }
+    // Keys inside a 4x4x4-block window anchored one block below the first live
+    // lane's point are marked in a 64-bit mask (one cell per lane later) with
+    // no buffering; only keys outside it go through the per-lane buffer.
+    const unsigned long long live = __ballot(ok);
+    if (live == 0) return;
+    const int leader = __ffsll((long long)live) - 1;
+    const int32_t ox = __builtin_amdgcn_readlane(floor_div8((int32_t)floorf(px * v.inv_voxel)), leader) - 1;
+    const int32_t oy = __builtin_amdgcn_readlane(floor_div8((int32_t)floorf(py * v.inv_voxel)), leader) - 1;
+    const int32_t oz = __builtin_amdgcn_readlane(floor_div8((int32_t)floorf(pz * v.inv_voxel)), leader) - 1;
+    unsigned long long cells = 0;
     for (int32_t s = -v.steps; s <= v.steps; ++s) {
         if (ok) {
             const float t = (float)s * v.voxel;
@@ -358,10 +370,19 @@ __device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t*
                 const float sdf = dist - sqrtf(ex * ex + ey * ey + ez * ez);
                 culled = sdf < -v.trunc;
             }
-            if (!culled) lk.push(pack_coord(floor_div8(vx), floor_div8(vy), floor_div8(vz)));
+            if (!culled) {
+                const int32_t bx = floor_div8(vx), by = floor_div8(vy), bz = floor_div8(vz);
+                const uint32_t dx = uint32_t(bx - ox), dy = uint32_t(by - oy), dz = uint32_t(bz - oz);
+                if ((dx | dy | dz) < 4u) cells |= 1ull << (dx | (dy << 2) | (dz << 4));
+                else lk.push(pack_coord(bx, by, bz));
+            }
         }
         if (__ballot(lk.valid >> (kKeysPerLane - 1))) flush_wave<MODE, TOUCH>(c, lk);
     }
+    cells = wave_or_u64(cells);
+    const int lane = __lane_id();
+    process_keys<MODE, TOUCH>(c, (cells >> lane) & 1ull,
+                              pack_coord(ox + (lane & 3), oy + ((lane >> 2) & 3), oz + (lane >> 4)));
     flush_wave<MODE, TOUCH>(c, lk);
 }
 

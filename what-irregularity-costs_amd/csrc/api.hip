@@ -122,7 +122,7 @@ bool OsnTsdfHipVolume::alloc() {
         HIP_OK(hipMalloc(&d.vox, n_vox * voxel_bytes(d.storage)));
     }
     HIP_OK(hipMalloc(&d.new_keys, (size_t)cap * sizeof(int64_t)));
-    HIP_OK(hipMalloc(&d.touched, (size_t)cap * sizeof(uint32_t)));
+    HIP_OK(hipMalloc(&d.touched, (size_t)cap * sizeof(TouchEntry)));
     HIP_OK(hipMalloc(&d.sort_a, (size_t)cap * sizeof(int64_t)));
     HIP_OK(hipMalloc(&d.sort_b, (size_t)cap * sizeof(int64_t)));
     const size_t n_flags = (size_t)kMaxSortLevels * sort_flag_stride(cap);

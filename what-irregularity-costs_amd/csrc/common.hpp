@@ -76,6 +76,17 @@ struct Counters {
     ListCounters lists[2];
 };
 
+// One block a call visited: its key, its table slot and its pool index when
+// the block already existed (kIdxPending for blocks the call inserted, whose
+// index the assignment pass writes to the table), so the update reaches the
+// voxel record without walking slot -> index -> coordinate.
+struct TouchEntry {
+    int64_t key;
+    uint32_t slot;
+    int32_t bi;
+};
+static_assert(sizeof(TouchEntry) == 16, "touch entry must stay 16 bytes");
+
 // Everything a kernel needs to address one volume. Passed by value.
 struct VolDev {
     HashEntry* table;
@@ -103,7 +114,7 @@ struct VolDev {
     int32_t w_max;         // projective weight saturation
     // per-call work lists
     int64_t* new_keys;     // capacity entries
-    uint32_t* touched;     // capacity entries (slots)
+    struct TouchEntry* touched;  // capacity entries
     int64_t* sort_a;       // capacity entries
     int64_t* sort_b;       // capacity entries
     int32_t* sort_flags;   // merge-tree arrival counters

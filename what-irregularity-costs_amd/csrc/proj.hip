@@ -35,12 +35,19 @@ namespace {
 // Everything the update reads for one scene, gathered once per launch (kernel
 // arguments in the single-volume kernel, LDS in the batched one) so that the
 // per-voxel loop never re-reads descriptors from global memory.
+#define OSN_GLOBAL __attribute__((address_space(1)))
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
 struct UpdParams {
-    const HashEntry* table;
-    const int32_t* block_coord;
-    void* vox;
-    const uint32_t* touched;
-    const float* depth;
+    // global address space, so loads through pointers kept in LDS stay
+    // global (not flat) loads
+    const HashEntry OSN_GLOBAL* table;
+    const int32_t OSN_GLOBAL* block_coord;
+    char OSN_GLOBAL* vox;
+    const TouchEntry OSN_GLOBAL* touched;
+    const float OSN_GLOBAL* depth;
     float Rcw[9];
     float c[3];
     float fx, fy, cx, cy;
@@ -54,11 +61,11 @@ struct UpdParams {
 
 __device__ inline UpdParams make_params(const VolDev& v, const ProjCam& k, int parity) {
     UpdParams p;
-    p.table = v.table;
-    p.block_coord = v.block_coord;
-    p.vox = v.vox;
-    p.touched = v.touched;
-    p.depth = k.depth;
+    p.table = (const HashEntry OSN_GLOBAL*)v.table;
+    p.block_coord = (const int32_t OSN_GLOBAL*)v.block_coord;
+    p.vox = (char OSN_GLOBAL*)v.vox;
+    p.touched = (const TouchEntry OSN_GLOBAL*)v.touched;
+    p.depth = (const float OSN_GLOBAL*)k.depth;
     for (int i = 0; i < 9; ++i) { p.Rcw[i] = k.Rcw[i]; p.Rcw64[i] = k.Rcw64[i]; }
     for (int i = 0; i < 3; ++i) { p.c[i] = k.c[i]; p.c64[i] = k.c64[i]; }
     p.fx = k.fx; p.fy = k.fy; p.cx = k.cx; p.cy = k.cy;
@@ -92,23 +99,23 @@ __device__ inline void block_frame(const UpdParams& k, int32_t bx, int32_t by, i
     }
 }
 
-// FP32 decision for the voxel at camera point p. Returns false when it gets no
-// observation; otherwise the pixel's depth and the FP32 sdf.
+// FP32 decision for the voxel at camera point p: true when it gets an
+// observation, with the pixel's depth and the FP32 sdf. Evaluated without
+// branches (an out-of-image voxel reads pixel 0 and is then rejected), so a
+// wave never splits on it.
 __device__ inline bool observe(const UpdParams& k, float p0, float p1, float p2, float& sdf_out,
                                float& depth_out) {
-    if (!(p2 > 0.0f)) return false;
     const float iz = 1.0f / p2;
     const float uf = k.fx * (p0 * iz) + k.cx + 0.5f;
     const float vf = k.fy * (p1 * iz) + k.cy + 0.5f;
-    if (!(uf >= 0.0f) || !(vf >= 0.0f) || !(uf < (float)k.W) || !(vf < (float)k.H)) return false;
-    const int u = (int)uf, w = (int)vf;  // uf, vf >= 0: truncation is floor
+    const bool inb = (p2 > 0.0f) & (uf >= 0.0f) & (vf >= 0.0f) & (uf < (float)k.W) &
+                     (vf < (float)k.H);
+    const int u = inb ? (int)uf : 0, w = inb ? (int)vf : 0;  // uf, vf >= 0: truncation is floor
     const float d = k.depth[w * k.W + u];
-    if (!(d > 0.0f)) return false;
     const float sdf = d - p2;
-    if (sdf < -k.trunc) return false;
     sdf_out = sdf;
     depth_out = d;
-    return true;
+    return inb & (d > 0.0f) & !(sdf < -k.trunc);
 }
 
 __device__ inline float avg_step(float t, float w, float x) {
@@ -126,17 +133,19 @@ struct Layout<kStoreF32> { static constexpr int kVpl = 2; };
 template <>
 struct Layout<kStoreF64> { static constexpr int kVpl = 1; };
 
-// One wave updates the whole block stored at table slot `slot` (uniform over
-// the wave). Load i of lane l covers voxels [i*64*V + l*V, +V).
+// One wave updates the whole block of touch entry `item` (uniform over the
+// wave). Load i of lane l covers voxels [i*64*V + l*V, +V).
 template <int STORE>
-__device__ inline void update_block(const UpdParams& k, uint32_t slot, int lane) {
+__device__ inline void update_block(const UpdParams& k, int item, int lane) {
     constexpr int V = Layout<STORE>::kVpl;
     constexpr int L = kBlockVoxels / (64 * V);
-    const int32_t bi = k.table[slot].block_idx;
+    // {int64 key, uint32 slot, int32 bi} in one 16-byte load
+    const u32x4 e = *reinterpret_cast<const u32x4 OSN_GLOBAL*>(k.touched + item);
+    int32_t bi = (int32_t)e.w;
+    if (bi == kIdxPending) bi = k.table[e.z].block_idx;  // inserted by this call
     if (bi < 0) return;
-    const int32_t bx = k.block_coord[bi * 3 + 0];
-    const int32_t by = k.block_coord[bi * 3 + 1];
-    const int32_t bz = k.block_coord[bi * 3 + 2];
+    int32_t bx, by, bz;
+    unpack_key((int64_t)(((uint64_t)e.y << 32) | e.x), bx, by, bz);
     BlockFrame f;
     block_frame(k, bx, by, bz, f);
 
@@ -150,54 +159,55 @@ __device__ inline void update_block(const UpdParams& k, uint32_t slot, int lane)
         for (int c = 0; c < 3; ++c) row[c] = (f.d0[c] + fly * f.e[1][c]) + flz * f.e[2][c];
 
         if constexpr (STORE == kStoreF16) {
-            uint4* rec = reinterpret_cast<uint4*>(static_cast<uint32_t*>(k.vox) +
-                                                  (size_t)bi * kBlockVoxels + vi0);
-            const uint4 q = *rec;
+            u32x4 OSN_GLOBAL* rec = reinterpret_cast<u32x4 OSN_GLOBAL*>(
+                k.vox + ((size_t)bi * kBlockVoxels + vi0) * 4);
+            const u32x4 q = *rec;
             uint32_t word[4] = {q.x, q.y, q.z, q.w};
             bool dirty = false;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float flx = (float)(lx0 + j);
                 float sdf, d;
-                if (!observe(k, row[0] + flx * f.e[0][0], row[1] + flx * f.e[0][1],
-                             row[2] + flx * f.e[0][2], sdf, d))
-                    continue;
+                const bool ok = observe(k, row[0] + flx * f.e[0][0], row[1] + flx * f.e[0][1],
+                                        row[2] + flx * f.e[0][2], sdf, d);
                 const float x = fminf(sdf * k.inv_trunc, 1.0f);
                 const float tt = __half2float(__ushort_as_half((unsigned short)(word[j] & 0xffffu)));
                 const float w = (float)(word[j] >> 16);
                 const float tn = avg_step(tt, w, x);
                 const float wn = fminf(w + 1.0f, k.wmax);
-                word[j] = (uint32_t)__half_as_ushort(__float2half_rn(tn)) | ((uint32_t)wn << 16);
-                dirty = true;
+                const uint32_t nw = (uint32_t)__half_as_ushort(__float2half_rn(tn)) | ((uint32_t)wn << 16);
+                word[j] = ok ? nw : word[j];
+                dirty |= ok;
             }
-            if (dirty) *rec = make_uint4(word[0], word[1], word[2], word[3]);
+            if (dirty) *rec = u32x4{word[0], word[1], word[2], word[3]};
         } else if constexpr (STORE == kStoreF32) {
-            float4* rec = reinterpret_cast<float4*>(static_cast<float*>(k.vox) +
-                                                    ((size_t)bi * kBlockVoxels + vi0) * 2);
-            const float4 q = *rec;
+            f32x4 OSN_GLOBAL* rec = reinterpret_cast<f32x4 OSN_GLOBAL*>(
+                k.vox + ((size_t)bi * kBlockVoxels + vi0) * 8);
+            const f32x4 q = *rec;
             float tv[2] = {q.x, q.z}, wv[2] = {q.y, q.w};
             bool dirty = false;
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const float flx = (float)(lx0 + j);
                 float sdf, d;
-                if (!observe(k, row[0] + flx * f.e[0][0], row[1] + flx * f.e[0][1],
-                             row[2] + flx * f.e[0][2], sdf, d))
-                    continue;
+                const bool ok = observe(k, row[0] + flx * f.e[0][0], row[1] + flx * f.e[0][1],
+                                        row[2] + flx * f.e[0][2], sdf, d);
                 const float x = fminf(sdf * k.inv_trunc, 1.0f);
-                tv[j] = avg_step(tv[j], wv[j], x);
-                wv[j] = fminf(wv[j] + 1.0f, k.wmax);
-                dirty = true;
+                const float tn = avg_step(tv[j], wv[j], x);
+                const float wn = fminf(wv[j] + 1.0f, k.wmax);
+                tv[j] = ok ? tn : tv[j];
+                wv[j] = ok ? wn : wv[j];
+                dirty |= ok;
             }
-            if (dirty) *rec = make_float4(tv[0], wv[0], tv[1], wv[1]);
+            if (dirty) *rec = f32x4{tv[0], wv[0], tv[1], wv[1]};
         } else {
-            double2* rec = reinterpret_cast<double2*>(static_cast<double*>(k.vox) +
-                                                      ((size_t)bi * kBlockVoxels + vi0) * 2);
+            f64x2 OSN_GLOBAL* rec = reinterpret_cast<f64x2 OSN_GLOBAL*>(
+                k.vox + ((size_t)bi * kBlockVoxels + vi0) * 16);
             const float flx = (float)lx0;
             float sdf, d;
             if (observe(k, row[0] + flx * f.e[0][0], row[1] + flx * f.e[0][1],
                         row[2] + flx * f.e[0][2], sdf, d)) {
-                double2 q = *rec;
+                f64x2 q = *rec;
                 const double vs = (double)k.voxel;
                 const double wx = ((double)(bx * 8 + lx0) + 0.5) * vs - k.c64[0];
                 const double wy = ((double)(by * 8 + ly) + 0.5) * vs - k.c64[1];
@@ -222,7 +232,7 @@ __global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, Call
     const int lane = threadIdx.x & 63;
     const UpdParams k = make_params(v, cam, call.parity);
     for (int item = blockIdx.x * waves + wave_id(); item < k.n; item += gridDim.x * waves)
-        update_block<STORE>(k, k.touched[item], lane);
+        update_block<STORE>(k, item, lane);
 }
 
 // All scenes of a batched step in one launch: the work list is the
@@ -265,7 +275,7 @@ __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
          item += per_xcd * waves) {
         while (s_prefix[sc + 1] <= item) ++sc;  // items ascend, so the scene only moves forward
         const UpdParams& p = s_p[sc];
-        update_block<STORE>(p, p.touched[item - s_prefix[sc]], lane);
+        update_block<STORE>(p, item - s_prefix[sc], lane);
     }
 }
 
