@@ -131,17 +131,25 @@ def run_batched(H, cfg, storage, scenes, warmup, steps, graph=True):
 
 
 def run_batched_instrumented(H, cfg, storage, scenes, warmup, steps, bytes_per_voxel):
-    """Eager batched steps with HIP events per stage and per-step touched counts."""
+    """Per-stage HIP-event times of back-to-back eager batched steps, and the
+    per-step touched-block counts (the same frames give the same touched sets,
+    so they are collected in a second, synchronised pass that is not timed)."""
     b, vols = make_batch(H, cfg, storage, scenes)
     b.run(warmup)
     b.synchronize()
     b.read_timing()
     b.set_timing(True)
+    b.run(steps)
+    ms, calls = b.read_timing()
+    b.close()
+    for v in vols:
+        v.close()
+    b, vols = make_batch(H, cfg, storage, scenes)
+    b.run(warmup)
     touched = []
     for _ in range(steps):
         b.run(1)
         touched.append(sum(v.touched_count for v in vols))  # synchronises
-    ms, calls = b.read_timing()
     W, Hh = cfg["width"], cfg["height"]
     tb = np.array(touched, np.float64)
     alg_bytes = tb * 512 * 2 * bytes_per_voxel + len(scenes) * W * Hh * 4 + tb * 12
@@ -231,6 +239,9 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="library default launch shape")
+    ap.add_argument("--knobs", default=None,
+                    help="fixed launch shape as JSON (alloc_grid, lds_set, hash_kind, "
+                         "update_threads, update_grid); skips the autotuner")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the batched fp16 run and one 256 MiB device copy (the copy "
                          "calibrates HBM byte counters), for rocprofv3 passes")
@@ -252,7 +263,10 @@ def main():
     frames_timed = args.steps * args.scenes
 
     tune_log = None
-    if not args.no_tune:
+    if args.knobs:
+        KNOBS.update(json.loads(args.knobs))
+        tune_log = {"chosen": dict(KNOBS), "trials": 0, "seconds": 0.0}
+    elif not args.no_tune:
         import autotune
         log("autotuning the launch shape on the first frames")
         n_t = min(n, 30)
