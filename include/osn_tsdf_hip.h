@@ -185,6 +185,17 @@ int32_t osn_tsdf_hip_backproject(const float* d_depth, const OsnTsdfHipFrame* fr
 /* Insert block coordinates directly (int32 x,y,z per key, device pointer):
  * the allocate path without the band walk, for allocator stress tests. */
 int32_t osn_tsdf_hip_insert_blocks(OsnTsdfHipVolume* v, const int32_t* d_coords, int32_t n);
+/* insert_blocks with the allocate measurement modes: 0 plain, 1 count
+ * compare-exchange attempts (osn_tsdf_hip_cas_attempts), 2 probe-depth
+ * statistics (osn_tsdf_hip_probe_stats). */
+int32_t osn_tsdf_hip_insert_blocks_mode(OsnTsdfHipVolume* v, const int32_t* d_coords, int32_t n,
+                                        int32_t mode);
+/* Total compare-exchange attempts counted since create/reset (synchronises). */
+int32_t osn_tsdf_hip_cas_attempts(const OsnTsdfHipVolume* v, uint64_t* out);
+/* Uncontended 64-bit compare-exchange ceiling of the device: 2^log2_slots
+ * CASes, one per slot in a scattered order, averaged over `reps` timed runs;
+ * milliseconds per run into *ms. Allocates and frees its own buffer. */
+int32_t osn_tsdf_hip_cas_ceiling(int32_t log2_slots, int32_t reps, double* ms);
 
 /* Distinct blocks the last allocate call visited (synchronises). */
 int32_t osn_tsdf_hip_touched_count(const OsnTsdfHipVolume* v);

@@ -1,0 +1,67 @@
+"""Allocator stress on the GPU: 2^22 Zipf-shuffled inserts over 2^18 keys into
+tables at load factors 0.5 / 0.7 / 0.9 (non-power-of-two capacities, mixed
+hash) and the reference hash at 0.5. The key set must equal the universe (the
+closed-form oracle) exactly, nothing may drop, and the pool order is sorted.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def data():
+    import stress
+    uni, stream = stress.make_stream(seed=11)
+    return stress, uni, stream
+
+
+@pytest.mark.parametrize("label,lf,hk", [("0.5", 0.5, 1), ("0.7", 0.7, 1), ("0.9", 0.9, 1),
+                                         ("0.5-ref", 0.5, 0)])
+def test_stress_key_set_exact(data, label, lf, hk):
+    import osn_tsdf_hip as H
+    stress, uni, stream = data
+    K = len(uni)
+    cap = stress.capacity(K, lf, hk)
+    v = H.HipVolume(0.01, K, -1.0, 0.0, storage=H.STORE_F16, table_slots=cap, hash_kind=hk)
+    d = torch.from_numpy(stream.ravel()).cuda()
+    assert v.insert_blocks(d) == 0
+    assert v.block_count == K
+    assert v.drop_count == 0
+    keys = v.keys()
+    assert np.all(np.diff(keys) > 0)  # pool order = sorted keys
+    assert np.array_equal(keys, np.sort(stress.pack_keys(uni)))
+    # inserting the stream again finds everything and adds nothing
+    assert v.insert_blocks(d) == 0
+    assert v.block_count == K and v.drop_count == 0
+
+
+def test_probe_histogram_consistent(data):
+    import osn_tsdf_hip as H
+    stress, uni, stream = data
+    K = len(uni)
+    d = torch.from_numpy(stream.ravel()).cuda()
+    means = []
+    for lf in (0.5, 0.9):
+        v = H.HipVolume(0.01, K, -1.0, 0.0, storage=H.STORE_F16,
+                        table_slots=stress.capacity(K, lf, 1), hash_kind=1)
+        assert H.insert_blocks_mode(v, d, 2) == 0
+        hist, lane, wave, lookups = v.probe_stats()
+        assert int(hist.sum()) == lookups >= K
+        assert wave >= lane >= lookups
+        means.append(lane / lookups)
+        assert np.array_equal(v.keys(), np.sort(stress.pack_keys(uni)))
+    assert means[1] > means[0]
+
+
+def test_counting_cas_and_ceiling(data):
+    import osn_tsdf_hip as H
+    stress, uni, stream = data
+    K = len(uni)
+    v = H.HipVolume(0.01, K, -1.0, 0.0, storage=H.STORE_F16,
+                    table_slots=stress.capacity(K, 0.7, 1), hash_kind=1)
+    d = torch.from_numpy(stream.ravel()).cuda()
+    assert H.insert_blocks_mode(v, d, 1) == 0
+    assert H.cas_attempts(v) >= K
+    assert H.cas_ceiling(16, reps=2) > 0

@@ -289,9 +289,13 @@ __device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t*
     const VolDev& v = *c.v;
     LaneKeys lk;
     if (SRC == kSrcCoords) {
+        // Given keys have no spatial coherence to exploit within a wave, so
+        // each lane goes straight to the workgroup pre-hash with its key.
         const int i = tile * kAllocThreads + threadIdx.x;
-        if (i < src.n) lk.push(pack_coord(coords[i * 3 + 0], coords[i * 3 + 1], coords[i * 3 + 2]));
-        flush_wave<MODE, TOUCH>(c, lk);
+        const bool has = i < src.n;
+        const int64_t key =
+            has ? pack_coord(coords[i * 3 + 0], coords[i * 3 + 1], coords[i * 3 + 2]) : kEmptyKey;
+        process_keys<MODE, TOUCH>(c, has, key);
         return;
     }
     bool ok = false;
@@ -810,6 +814,23 @@ hipError_t launch_backproject(const PointSrc& src, float* out, hipStream_t s) {
 
 hipError_t launch_step_advance(int32_t* step_ctr, hipStream_t s) {
     k_step_advance<<<1, 1, 0, s>>>(step_ctr);
+    return hipGetLastError();
+}
+
+// Uncontended 64-bit compare-exchange ceiling: op i swaps slot
+// (i * 2654435761) mod n_slots (a bijection when n_slots is a power of two),
+// so no two lanes in flight touch the same address and the addresses are
+// scattered over the whole buffer, as table inserts are.
+__global__ void k_cas_ceiling(unsigned long long* slots, uint32_t mask, int32_t n_ops) {
+    const int32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_ops) return;
+    const uint32_t s = (uint32_t)((uint64_t)(uint32_t)i * 2654435761ull) & mask;
+    atomicCAS(&slots[s], ~0ull, (unsigned long long)i);
+}
+
+hipError_t launch_cas_ceiling(unsigned long long* slots, uint32_t mask, int32_t n_ops,
+                              hipStream_t s) {
+    k_cas_ceiling<<<(n_ops + 255) / 256, 256, 0, s>>>(slots, mask, n_ops);
     return hipGetLastError();
 }
 

@@ -498,11 +498,57 @@ int32_t osn_tsdf_hip_backproject(const float* d_depth, const OsnTsdfHipFrame* fr
 }
 
 int32_t osn_tsdf_hip_insert_blocks(OsnTsdfHipVolume* v, const int32_t* d_coords, int32_t n) {
-    if (!v || !d_coords || n <= 0) return 1;
+    return osn_tsdf_hip_insert_blocks_mode(v, d_coords, n, 0);
+}
+
+int32_t osn_tsdf_hip_insert_blocks_mode(OsnTsdfHipVolume* v, const int32_t* d_coords, int32_t n,
+                                        int32_t mode) {
+    if (!v || !d_coords || n <= 0 || mode < 0 || mode > 2) return 1;
     if (!v->ok) return 2;
     PointSrc s{};
     s.n = n;
-    return run_allocate(v, s, kAllocNormal, false, d_coords);
+    const int m = mode == 1 ? kAllocCountCas : (mode == 2 ? kAllocProbe : kAllocNormal);
+    return run_allocate(v, s, m, false, d_coords);
+}
+
+int32_t osn_tsdf_hip_cas_attempts(const OsnTsdfHipVolume* v, uint64_t* out) {
+    if (!v || !out || !v->ok) return 1;
+    unsigned long long n = 0;
+    if (read_counter(v, offsetof(Counters, cas_attempts), &n)) return 2;
+    *out = n;
+    return 0;
+}
+
+int32_t osn_tsdf_hip_cas_ceiling(int32_t log2_slots, int32_t reps, double* ms_per_op_batch) {
+    if (log2_slots < 10 || log2_slots > 30 || reps <= 0 || !ms_per_op_batch) return 1;
+    const uint32_t n = 1u << log2_slots;
+    unsigned long long* slots = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s = nullptr;
+    int32_t rc = 0;
+    if (hipMalloc(&slots, (size_t)n * 8) != hipSuccess) return 2;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        rc = 2;
+    } else {
+        float total = 0.0f;
+        for (int r = 0; r <= reps && rc == 0; ++r) {  // r == 0 is a warmup
+            if (hipMemsetAsync(slots, 0xff, (size_t)n * 8, s) != hipSuccess) { rc = 2; break; }
+            hipEventRecord(a, s);
+            if (launch_cas_ceiling(slots, n - 1, (int32_t)n, s) != hipSuccess) { rc = 2; break; }
+            hipEventRecord(b, s);
+            float ms = 0.0f;
+            if (hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(&ms, a, b) != hipSuccess) { rc = 2; break; }
+            if (r > 0) total += ms;
+        }
+        *ms_per_op_batch = total / reps;
+    }
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    if (s) hipStreamDestroy(s);
+    hipFree(slots);
+    return rc;
 }
 
 int32_t osn_tsdf_hip_extract_mesh_device(OsnTsdfHipVolume* v, float* d_posnor, uint8_t* d_rgb,

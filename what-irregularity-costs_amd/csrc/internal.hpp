@@ -114,6 +114,8 @@ hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t 
 hipError_t launch_step_advance(int32_t* step_ctr, hipStream_t s);
 hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipStream_t s);
 hipError_t launch_fill_table(HashEntry* table, uint32_t n, hipStream_t s);
+hipError_t launch_cas_ceiling(unsigned long long* slots, uint32_t mask, int32_t n_ops,
+                              hipStream_t s);
 
 __host__ __device__ inline int32_t sort_flag_stride(uint32_t capacity) {
     return (int32_t)((capacity + kSortTile - 1) / kSortTile / 2 + 1);

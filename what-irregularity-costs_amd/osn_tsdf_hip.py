@@ -378,3 +378,38 @@ class Batch:
         calls = np.zeros(4, np.int64)
         self._L.osn_tsdf_hip_batch_read_timing(self.h, _hptr(ms), _hptr(calls))
         return ms, calls
+
+
+def _bind_extra(L):
+    extra = {
+        "osn_tsdf_hip_insert_blocks_mode": (_I, [_VP, _VP, _I, _I]),
+        "osn_tsdf_hip_cas_attempts": (_I, [_VP, _VP]),
+        "osn_tsdf_hip_cas_ceiling": (_I, [_I, _I, _VP]),
+    }
+    for name, (res, args) in extra.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def insert_blocks_mode(vol: HipVolume, d_coords, mode: int) -> int:
+    L = _bind_extra(lib())
+    return L.osn_tsdf_hip_insert_blocks_mode(vol.h, _dptr(d_coords), d_coords.numel() // 3, mode)
+
+
+def cas_attempts(vol: HipVolume) -> int:
+    L = _bind_extra(lib())
+    out = C.c_uint64(0)
+    if L.osn_tsdf_hip_cas_attempts(vol.h, C.addressof(out)):
+        raise RuntimeError("cas_attempts failed")
+    return out.value
+
+
+def cas_ceiling(log2_slots: int, reps: int = 5) -> float:
+    """Milliseconds for 2^log2_slots uncontended, scattered 64-bit CASes."""
+    L = _bind_extra(lib())
+    ms = C.c_double(0.0)
+    if L.osn_tsdf_hip_cas_ceiling(log2_slots, reps, C.addressof(ms)):
+        raise RuntimeError("cas_ceiling failed")
+    return ms.value
