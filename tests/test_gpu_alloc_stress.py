@@ -65,3 +65,47 @@ def test_counting_cas_and_ceiling(data):
     assert H.insert_blocks_mode(v, d, 1) == 0
     assert H.cas_attempts(v) >= K
     assert H.cas_ceiling(16, reps=2) > 0
+
+
+def _coords(rng, n, lim=1 << 10):
+    return rng.integers(-lim, lim, size=(n, 3), dtype=np.int32)
+
+
+@pytest.mark.parametrize("sizes", [(300, 5000), (20000, 9000, 1), (70000, 4097)])
+def test_consecutive_calls_pool_order(sizes):
+    """Each call's new keys get indices bc0, bc0+1, ... in key order, for calls
+    whose new-key counts need one tile, bitonic tiles and multi-level merges,
+    on both counter sets (consecutive calls)."""
+    import stress
+    import osn_tsdf_hip as H
+    rng = np.random.default_rng(5)
+    v = H.HipVolume(0.01, 1 << 17, -1.0, 0.0, storage=H.STORE_F16, table_slots=1 << 18,
+                    hash_kind=1)
+    seen = set()
+    expect = []
+    for n in sizes:
+        c = _coords(rng, n)
+        assert v.insert_blocks(torch.from_numpy(c.ravel()).cuda()) == 0
+        k = stress.pack_keys(c)
+        new = np.unique(k[~np.isin(k, np.fromiter(seen, np.int64, len(seen)))])
+        expect.extend(new.tolist())
+        seen.update(new.tolist())
+        got = v.keys()
+        assert v.block_count == len(expect)
+        assert np.array_equal(got, np.array(expect, np.int64))
+
+
+def test_extreme_key_sorts_last():
+    """Block (2^20-1)^3 packs to INT64_MAX, the value the in-LDS sort pads
+    with; it must survive the sort as the largest key."""
+    import stress
+    import osn_tsdf_hip as H
+    rng = np.random.default_rng(9)
+    c = np.concatenate([_coords(rng, 700), np.array([[(1 << 20) - 1] * 3, [-(1 << 20)] * 3],
+                                                     np.int32)])
+    v = H.HipVolume(0.01, 4096, -1.0, 0.0, storage=H.STORE_F16, table_slots=1 << 13,
+                    hash_kind=1)
+    assert v.insert_blocks(torch.from_numpy(c.ravel()).cuda()) == 0
+    want = np.unique(stress.pack_keys(c))
+    assert want[-1] == np.iinfo(np.int64).max
+    assert np.array_equal(v.keys(), want)

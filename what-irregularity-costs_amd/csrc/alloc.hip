@@ -27,7 +27,8 @@ namespace {
 
 constexpr int kAllocThreads = 256;  // 4 waves; 16x16 pixels in depth mode
 constexpr int kKeysPerLane = 8;     // distinct keys a lane buffers before a flush
-constexpr int kWgList = 512;        // LDS list capacity per workgroup
+constexpr int kWgList = 512;        // LDS touched-list capacity per workgroup
+constexpr int kWgNew = 256;         // LDS new-key-list capacity per workgroup
 constexpr int kMaxSet = 1024;       // LDS pre-hash capacity
 
 enum SrcKind : int { kSrcPoints = 0, kSrcDepth = 1, kSrcCoords = 2 };
@@ -50,7 +51,7 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long x) {
 
 struct WgShared {
     int64_t set[kMaxSet];
-    int64_t new_keys[kWgList];
+    TouchEntry new_keys[kWgNew];
     TouchEntry touched[kWgList];
     int32_t n_new, n_touched;
     int32_t base_new, base_touched;
@@ -60,7 +61,7 @@ struct WgShared {
 
 // Append `mine` (when `want`) to an LDS list, one LDS atomic per wave; past
 // the LDS capacity, straight to the global list.
-template <typename T>
+template <int CAP, typename T>
 __device__ inline void wave_append(bool want, T mine, int32_t* lds_count, T* lds_list,
                                    int32_t* g_count, T* g_list) {
     const unsigned long long m = __ballot(want);
@@ -71,7 +72,7 @@ __device__ inline void wave_append(bool want, T mine, int32_t* lds_count, T* lds
     base = __builtin_amdgcn_readlane(base, leader);
     if (!want) return;
     const int i = base + __popcll(m & lanemask_lt());
-    if (i < kWgList) lds_list[i] = mine;
+    if (i < CAP) lds_list[i] = mine;
     else g_list[atomicAdd(g_count, 1)] = mine;
 }
 
@@ -79,12 +80,12 @@ __device__ inline void wave_append(bool want, T mine, int32_t* lds_count, T* lds
 __device__ void flush_lists(const VolDev& v, ListCounters* lc, WgShared& s) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int nt = min(s.n_touched, kWgList), nn = min(s.n_new, kWgList);
+        const int nt = min(s.n_touched, kWgList), nn = min(s.n_new, kWgNew);
         s.base_touched = nt ? atomicAdd(&lc->touched_count, nt) : 0;
         s.base_new = nn ? atomicAdd(&lc->new_count, nn) : 0;
     }
     __syncthreads();
-    const int nt = min(s.n_touched, kWgList), nn = min(s.n_new, kWgList);
+    const int nt = min(s.n_touched, kWgList), nn = min(s.n_new, kWgNew);
     for (int i = threadIdx.x; i < nt; i += blockDim.x) v.touched[s.base_touched + i] = s.touched[i];
     for (int i = threadIdx.x; i < nn; i += blockDim.x) v.new_keys[s.base_new + i] = s.new_keys[i];
     __syncthreads();
@@ -210,10 +211,11 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
         touch = (old != c.stamp);
     }
     if (TOUCH)
-        wave_append<TouchEntry>(touch, TouchEntry{key, slot, idx}, &s.n_touched, s.touched,
-                                &c.lc->touched_count, v.touched);
+        wave_append<kWgList>(touch, TouchEntry{key, slot, idx}, &s.n_touched, s.touched,
+                             &c.lc->touched_count, v.touched);
     if (MODE != kAllocNoCas)
-        wave_append<int64_t>(inserted, key, &s.n_new, s.new_keys, &c.lc->new_count, v.new_keys);
+        wave_append<kWgNew>(inserted, TouchEntry{key, slot, kIdxPending}, &s.n_new, s.new_keys,
+                            &c.lc->new_count, v.new_keys);
 }
 
 // Drain every lane's buffered keys: repeatedly take the first live key of the
@@ -434,6 +436,7 @@ __device__ void flush_stats(const VolDev& v, WgShared& s) {
 __device__ inline void reset_other_parity(Counters* ctr, int parity) {
     ctr->lists[parity ^ 1].new_count = 0;
     ctr->lists[parity ^ 1].touched_count = 0;
+    ctr->lists[parity ^ 1].assign_ticket = 0;
 }
 
 template <int MODE, bool TOUCH, int SRC>
@@ -493,175 +496,316 @@ __global__ __launch_bounds__(kAllocThreads) void k_alloc_batch(BatchDev b, int32
 }
 
 // ---- phase B: sort the call's new keys, assign pool indices in key order ----
+//
+// The call's n new keys (each with the slot it was inserted at) form
+// T = ceil(n / kSortTile) tiles. Work item (0, c) sorts tile c in LDS; item
+// (l, c) with l >= 1 produces output chunk c (positions c * kSortTile ...) of
+// level l by merging the two level-(l - 1) runs that cover it. Items of the
+// top level L = ceil(log2 T) do not store their run: output position p
+// becomes pool index bc0 + p, written to the key's table slot and to
+// block_coord. Level l writes buffer l & 1 (sort_a, sort_b) and reads the
+// other one.
+//
+// Workgroups take items in ticket order (level-major) from a per-call counter.
+// An item waits only for items of the level below, whose tickets are smaller
+// and so were claimed by workgroups already running; the lowest unfinished
+// ticket can always proceed, hence every wait ends. Each wait is bounded all
+// the same: past the bound the call records a fault, and every other waiter
+// sees it and leaves, so the kernel always drains.
 
-__device__ inline uint32_t find_slot(const VolDev& v, int64_t key) {
-    uint32_t s = home_slot(v, key);
-    for (uint32_t p = 0; p < v.capacity; ++p) {
-        if (v.table[s].key == key) return s;
-        s = next_slot(v, s);
-    }
-    return 0xffffffffu;
-}
+constexpr uint32_t kSpinLimit = 1u << 22;   // about 0.5 s of sleeps
 
-// Merge two sorted runs of distinct keys into dst with all threads of the
-// workgroup: each thread owns an equal slice of the output and finds where it
-// starts by binary search on the merge path.
-__device__ void merge_runs(const int64_t* a, int la, const int64_t* b, int lb, int64_t* dst) {
-    const int m = la + lb;
-    const int per = (m + blockDim.x - 1) / blockDim.x;
-    const int k0 = min(m, (int)threadIdx.x * per);
-    const int k1 = min(m, k0 + per);
-    if (k0 >= k1) return;
-    int lo = max(0, k0 - lb), hi = min(k0, la);
-    while (lo < hi) {  // i = elements taken from a among the first k0 outputs
-        const int i = (lo + hi) >> 1;
-        if (a[i] < b[k0 - i - 1]) lo = i + 1; else hi = i;
-    }
-    int i = lo, j = k0 - lo;
-    for (int k = k0; k < k1; ++k) {
-        const bool take_a = (j >= lb) || (i < la && a[i] < b[j]);
-        dst[k] = take_a ? a[i++] : b[j++];
-    }
-}
+struct AssignLds {
+    int64_t key[kSortTile];
+    uint32_t slot[kSortTile];
+    int32_t ticket;
+    int32_t flag;
+    int32_t corank[2];
+    unsigned long long drops;
+};
 
-// Sort s_keys[0, cnt) in LDS: rank sort for short lists (one pass, one
-// barrier), bitonic sort over the padded power of two otherwise. Keys are
-// distinct. Result in s_out (rank sort) or s_keys (bitonic); returns the array.
-__device__ int64_t* lds_sort(int64_t* s_keys, int64_t* s_out, int cnt) {
-    if (cnt <= (int)blockDim.x) {
+// Sort key/slot[0, cnt) in place: rank sort for short lists (one barrier),
+// bitonic over the padded power of two otherwise. Real keys are distinct; the
+// padding compares above any real entry through the slot tie-break, since a
+// real key may itself equal INT64_MAX.
+__device__ void lds_sort(AssignLds& s, int cnt) {
+    if (cnt <= 256) {
+        const bool has = (int)threadIdx.x < cnt;
         int64_t mine = 0;
+        uint32_t mslot = 0;
         int rank = 0;
-        if ((int)threadIdx.x < cnt) {
-            mine = s_keys[threadIdx.x];
-            for (int j = 0; j < cnt; ++j) rank += s_keys[j] < mine;
+        if (has) {
+            mine = s.key[threadIdx.x];
+            mslot = s.slot[threadIdx.x];
+            for (int j = 0; j < cnt; ++j) rank += s.key[j] < mine;
         }
         __syncthreads();
-        if ((int)threadIdx.x < cnt) s_out[rank] = mine;
+        if (has) { s.key[rank] = mine; s.slot[rank] = mslot; }
         __syncthreads();
-        return s_out;
+        return;
     }
-    int P = 2;
+    int P = 512;
     while (P < cnt) P <<= 1;
-    for (int i = cnt + threadIdx.x; i < P; i += blockDim.x) s_keys[i] = INT64_MAX;
+    for (int i = cnt + threadIdx.x; i < P; i += blockDim.x) {
+        s.key[i] = INT64_MAX;
+        s.slot[i] = 0xffffffffu;
+    }
     __syncthreads();
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < P; i += blockDim.x) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const int64_t x = s_keys[i], y = s_keys[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((x > y) == up) { s_keys[i] = y; s_keys[ixj] = x; }
+            for (int t = threadIdx.x; t < (P >> 1); t += blockDim.x) {
+                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));  // lower index of the pair
+                const int ixj = i + j;
+                const int64_t x = s.key[i], y = s.key[ixj];
+                const uint32_t sx = s.slot[i], sy = s.slot[ixj];
+                const bool gt = x > y || (x == y && sx > sy);
+                if (gt == ((i & k) == 0)) {
+                    s.key[i] = y; s.key[ixj] = x;
+                    s.slot[i] = sy; s.slot[ixj] = sx;
                 }
             }
             __syncthreads();
         }
     }
-    return s_keys;
 }
 
-__device__ void assign_sorted(const VolDev& v, const int64_t* keys, int n) {
-    const int bc0 = v.ctr->block_count;
-    unsigned long long drops = 0;
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        const int64_t key = keys[r];
-        const uint32_t slot = find_slot(v, key);
-        if (slot == 0xffffffffu) continue;
-        const int64_t idx = (int64_t)bc0 + r;
-        if (idx < v.pool_capacity) {
-            int32_t x, y, z;
-            unpack_key(key, x, y, z);
-            v.block_coord[idx * 3 + 0] = x;
-            v.block_coord[idx * 3 + 1] = y;
-            v.block_coord[idx * 3 + 2] = z;
-            v.table[slot].block_idx = (int32_t)idx;
-        } else {
-            v.table[slot].block_idx = kIdxDropped;
-            drops += 1;
-        }
+// Number of A entries among the first k outputs of merge(A, B), keys
+// distinct, by one wave: each round tests 64 evenly spaced candidates of the
+// monotone predicate A[i] < B[k - i - 1] and keeps the gap where it turns.
+__device__ int corank_wave(const TouchEntry* A, int la, const TouchEntry* B, int lb, int k) {
+    int lo = max(0, k - lb), hi = min(k, la);
+    const int lane = __lane_id();
+    while (lo < hi) {
+        const int stride = (hi - lo + 63) >> 6;
+        const int i = lo + lane * stride;
+        const bool p = i < hi && A[i].key < B[k - i - 1].key;
+        const int t = __popcll(__ballot(p));
+        if (t == 0) break;
+        const int nhi = min(hi, lo + t * stride);
+        lo = lo + (t - 1) * stride + 1;
+        hi = nhi;
     }
-    drops = wave_sum_u64(drops);
-    if ((threadIdx.x & 63) == 0 && drops) atomicAdd(&v.ctr->drop_count, drops);
+    return lo;
+}
+
+// Same in LDS, one thread, binary search.
+__device__ inline int corank_lds(const int64_t* a, int la, const int64_t* b, int lb, int k) {
+    int lo = max(0, k - lb), hi = min(k, la);
+    while (lo < hi) {
+        const int i = (lo + hi) >> 1;
+        if (a[i] < b[k - i - 1]) lo = i + 1; else hi = i;
+    }
+    return lo;
+}
+
+struct AssignCall {
+    int32_t n, T, L, t_stride;
+    int32_t* done;      // this parity's counters, level l at done[l * t_stride]
+    int32_t bc0;
+};
+
+// Output position p of the call's sorted order: store it, or at the top level
+// assign its pool index.
+__device__ inline void emit_sorted(const VolDev& v, const AssignCall& a, bool top, TouchEntry* dst,
+                                   int p, int64_t key, uint32_t slot, unsigned long long& drops) {
+    if (!top) {
+        dst[p] = TouchEntry{key, slot, kIdxPending};
+        return;
+    }
+    const int64_t idx = (int64_t)a.bc0 + p;
+    if (idx < v.pool_capacity) {
+        int32_t x, y, z;
+        unpack_key(key, x, y, z);
+        v.block_coord[idx * 3 + 0] = x;
+        v.block_coord[idx * 3 + 1] = y;
+        v.block_coord[idx * 3 + 2] = z;
+        v.table[slot].block_idx = (int32_t)idx;
+    } else {
+        v.table[slot].block_idx = kIdxDropped;
+        drops += 1;
+    }
+}
+
+// Thread 0 waits until *p >= target; false (for every thread) when the call
+// has faulted. The spin reads with relaxed loads; one agent-scope acquire
+// fence afterwards (an L1 + L2 invalidate on this XCD) makes the producers'
+// runs visible to every wave of the workgroup, which read only after the
+// barrier. An acquire load inside the spin would invalidate L2 on every
+// round and evict what concurrent items on the XCD are merging.
+__device__ bool wait_count(const VolDev& v, AssignLds& s, int32_t* p, int32_t target) {
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        for (uint32_t it = 0;; ++it) {
+            if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+            if (__hip_atomic_load(&v.ctr->fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
+            }
+            if (it >= kSpinLimit) {
+                atomicOr(&v.ctr->fault, 1u);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+        }
+        if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s.flag = ok;
+    }
     __syncthreads();
-    if (threadIdx.x == 0)
-        v.ctr->block_count = (int32_t)min((int64_t)bc0 + n, (int64_t)v.pool_capacity);
+    const bool ok = s.flag != 0;
+    __syncthreads();
+    return ok;
 }
 
-// Workgroups stride over 4096-key tiles; each sorts its tile in LDS, then
-// climbs a merge tree in which the second of two sibling runs to finish merges
-// them, so no workgroup ever waits on another. The one that completes the top
-// assigns indices bc0, bc0+1, ... in key order.
-__device__ void assign_volume(const VolDev& v, int parity, int32_t flag_stride, int wg, int n_wg,
-                              int64_t* s_keys, int64_t* s_tmp) {
-    __shared__ int s_arrival;
-    const int n = v.ctr->lists[parity].new_count;
-    if (n == 0) return;
-    const int n_tiles = (n + kSortTile - 1) / kSortTile;
-    for (int tile = wg; tile < n_tiles; tile += n_wg) {
-        const int lo = tile * kSortTile;
-        const int cnt = min(kSortTile, n - lo);
-        for (int i = threadIdx.x; i < cnt; i += blockDim.x) s_keys[i] = v.new_keys[lo + i];
-        __syncthreads();
-        int64_t* sorted = lds_sort(s_keys, s_tmp, cnt);
-        if (n_tiles == 1) {  // the common case: everything stays in LDS
-            assign_sorted(v, sorted, cnt);
-            return;
+// Every wave's global stores reach L2 (the barrier alone waits only for LDS),
+// then thread 0 writes this XCD's L2 back with one agent-scope release fence
+// and bumps *ctr. Returns the old count in thread 0.
+__device__ inline int publish_count(int32_t* ctr) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    int old = 0;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        old = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return old;
+}
+
+__device__ inline int32_t chunks_in_run(const AssignCall& a, int level, int run) {
+    return min((run + 1) << level, a.T) - (run << level);
+}
+
+// One work item; false when the call has faulted.
+__device__ bool assign_item(const VolDev& v, const AssignCall& a, AssignLds& s, int level, int c) {
+    const bool top = level == a.L;
+    TouchEntry* dst = (level & 1) ? v.sort_b : v.sort_a;
+    const TouchEntry* src = (level & 1) ? v.sort_a : v.sort_b;
+    const int o0 = c * kSortTile;
+    const int o1 = min(o0 + kSortTile, a.n);
+    unsigned long long drops = 0;
+    if (level == 0) {
+        const int cnt = o1 - o0;
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+            const TouchEntry e = v.new_keys[o0 + i];
+            s.key[i] = e.key;
+            s.slot[i] = e.slot;
         }
-        for (int i = threadIdx.x; i < cnt; i += blockDim.x) v.sort_a[lo + i] = sorted[i];
-        int64_t* src = v.sort_a;
-        int64_t* dst = v.sort_b;
-        int run = tile;
-        bool top = true;
-        for (int level = 0; (kSortTile << level) < n; ++level) {
-            const int width = kSortTile << level;
-            const int pair = run >> 1;
-            const int a_lo = pair * 2 * width;
-            const int a_hi = min(a_lo + width, n);
-            const int b_hi = min(a_lo + 2 * width, n);
-            if (a_hi >= b_hi) {
-                for (int i = a_lo + (int)threadIdx.x; i < a_hi; i += blockDim.x) dst[i] = src[i];
-            } else {
-                __threadfence();
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    int* flag = &v.sort_flags[level * flag_stride + pair];
-                    const int old = atomicAdd(flag, 1);
-                    if (old == 1) *flag = 0;  // reset for the next call
-                    s_arrival = old;
-                }
-                __syncthreads();
-                if (s_arrival == 0) { top = false; break; }  // sibling will merge
-                __threadfence();
-                merge_runs(src + a_lo, a_hi - a_lo, src + a_hi, b_hi - a_hi, dst + a_lo);
+        __syncthreads();
+        lds_sort(s, cnt);
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x)
+            emit_sorted(v, a, top, dst, o0 + i, s.key[i], s.slot[i], drops);
+    } else {
+        const int w_in = kSortTile << (level - 1);
+        const int q = c >> level;
+        const int a_lo = q * 2 * w_in;
+        const int a_hi = min(a_lo + w_in, a.n);
+        const int b_hi = min(a_lo + 2 * w_in, a.n);
+        int32_t* below = a.done + (level - 1) * a.t_stride;
+        if (!wait_count(v, s, below + 2 * q, chunks_in_run(a, level - 1, 2 * q))) return false;
+        if (a_hi < b_hi &&
+            !wait_count(v, s, below + 2 * q + 1, chunks_in_run(a, level - 1, 2 * q + 1)))
+            return false;
+        const int m = o1 - o0;
+        if (a_hi >= b_hi) {  // a lone run: copied (or assigned) as it is
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                const TouchEntry e = src[o0 + i];
+                emit_sorted(v, a, top, dst, o0 + i, e.key, e.slot, drops);
+            }
+        } else {
+            const TouchEntry* A = src + a_lo;
+            const TouchEntry* B = src + a_hi;
+            const int la = a_hi - a_lo, lb = b_hi - a_hi;
+            const int wave = threadIdx.x >> 6;
+            if (wave < 2) {
+                const int r = corank_wave(A, la, B, lb, (wave ? o1 : o0) - a_lo);
+                if (__lane_id() == 0) s.corank[wave] = r;
             }
             __syncthreads();
-            int64_t* t = src; src = dst; dst = t;
-            run = pair;
-        }
-        if (top) {
-            __threadfence();
+            const int i0 = s.corank[0], i1 = s.corank[1];
+            const int na = i1 - i0, nb = m - na;
+            const int j0 = (o0 - a_lo) - i0;
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                const TouchEntry e = i < na ? A[i0 + i] : B[j0 + i - na];
+                s.key[i] = e.key;
+                s.slot[i] = e.slot;
+            }
             __syncthreads();
-            assign_sorted(v, src, n);
+            const int per = (m + blockDim.x - 1) / blockDim.x;
+            const int k0 = min(m, (int)threadIdx.x * per);
+            const int k1 = min(m, k0 + per);
+            if (k0 < k1) {
+                const int64_t* ka = s.key;
+                const int64_t* kb = s.key + na;
+                int i = corank_lds(ka, na, kb, nb, k0);
+                int j = k0 - i;
+                for (int k = k0; k < k1; ++k) {
+                    const bool take_a = j >= nb || (i < na && ka[i] < kb[j]);
+                    const int from = take_a ? i++ : na + j++;
+                    emit_sorted(v, a, top, dst, o0 + k, s.key[from], s.slot[from], drops);
+                }
+            }
         }
+    }
+    // publish: this item's stores, then its count
+    if (top) {
+        drops = wave_sum_u64(drops);
+        if (__lane_id() == 0 && drops) atomicAdd(&s.drops, drops);
+    }
+    const int old = publish_count(a.done + level * a.t_stride + (c >> level));
+    if (threadIdx.x == 0) {
+        if (top && s.drops) atomicAdd(&v.ctr->drop_count, s.drops);
+        s.drops = 0;
+        s.flag = top && old == a.T - 1;  // the last top item finishes the call
+    }
+    __syncthreads();
+    if (s.flag) {
+        // every item has finished, so no one reads these counters any more
+        for (int l = 0; l <= a.L; ++l) {
+            const int runs = (a.T + (1 << l) - 1) >> l;
+            for (int r = threadIdx.x; r < runs; r += blockDim.x) a.done[l * a.t_stride + r] = 0;
+        }
+        if (threadIdx.x == 0)
+            v.ctr->block_count = (int32_t)min((int64_t)a.bc0 + a.n, (int64_t)v.pool_capacity);
+    }
+    __syncthreads();
+    return true;
+}
+
+__device__ void assign_volume(const VolDev& v, int parity, int wg, AssignLds& s) {
+    AssignCall a;
+    a.n = v.ctr->lists[parity].new_count;
+    if (a.n == 0) return;
+    a.T = (a.n + kSortTile - 1) / kSortTile;
+    a.L = ceil_log2(a.T);
+    if (wg >= a.T * (a.L + 1)) return;
+    a.t_stride = assign_tiles_max(v.capacity);
+    a.done = v.assign_done + (size_t)parity * assign_done_per_parity(v.capacity);
+    a.bc0 = v.ctr->block_count;
+    if (threadIdx.x == 0) s.drops = 0;
+    for (;;) {
         __syncthreads();
+        if (threadIdx.x == 0) s.ticket = atomicAdd(&v.ctr->lists[parity].assign_ticket, 1);
+        __syncthreads();
+        const int t = s.ticket;
+        if (t >= a.T * (a.L + 1)) break;
+        const int level = t / a.T;
+        if (!assign_item(v, a, s, level, t - level * a.T)) break;
     }
 }
 
-__global__ __launch_bounds__(1024) void k_assign(VolDev v, CallInfo call, int32_t flag_stride) {
-    __shared__ int64_t s_keys[kSortTile];
-    __shared__ int64_t s_tmp[1024];
-    assign_volume(v, call.parity, flag_stride, blockIdx.x, gridDim.x, s_keys, s_tmp);
+__global__ __launch_bounds__(1024) void k_assign(VolDev v, CallInfo call) {
+    __shared__ AssignLds s;
+    assign_volume(v, call.parity, blockIdx.x, s);
 }
 
 __global__ __launch_bounds__(1024) void k_assign_batch(BatchDev b) {
-    __shared__ int64_t s_keys[kSortTile];
-    __shared__ int64_t s_tmp[1024];
+    __shared__ AssignLds s;
     const int k = b.step_ctr ? *b.step_ctr : b.step;
     const int sc = blockIdx.y;
     if (!b.steps[k * b.n_scenes + sc].active) return;
     const VolDev v = b.vols[sc];
-    assign_volume(v, batch_call(b, sc, k).parity, sort_flag_stride(v.capacity), blockIdx.x,
-                  gridDim.x, s_keys, s_tmp);
+    assign_volume(v, batch_call(b, sc, k).parity, blockIdx.x, s);
 }
 
 // ---- reference-semantics scatter update (volume.cpp:200-299) ---------------
@@ -783,15 +927,17 @@ hipError_t launch_alloc_batch(const BatchDev& b, int32_t total_tiles, const Allo
 }
 
 hipError_t launch_assign(const VolDev& v, CallInfo call, hipStream_t s) {
-    const uint32_t tiles = (v.capacity + kSortTile - 1) / kSortTile;
+    const uint32_t tiles = (uint32_t)assign_tiles_max(v.capacity);
     const uint32_t grid = tiles < (uint32_t)kAssignMaxGrid ? tiles : (uint32_t)kAssignMaxGrid;
-    k_assign<<<grid, 1024, 0, s>>>(v, call, sort_flag_stride(v.capacity));
+    k_assign<<<grid, 1024, 0, s>>>(v, call);
     return hipGetLastError();
 }
 
 hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStream_t s) {
-    const uint32_t tiles = (max_capacity + kSortTile - 1) / kSortTile;
-    const uint32_t gx = tiles < 64u ? tiles : 64u;
+    // items are taken by ticket, so any width is correct; a scene's first
+    // frame is the only one with more than a few tiles of new keys
+    const uint32_t tiles = (uint32_t)assign_tiles_max(max_capacity);
+    const uint32_t gx = tiles < 32u ? tiles : 32u;
     k_assign_batch<<<dim3(gx, b.n_scenes), 1024, 0, s>>>(b);
     return hipGetLastError();
 }

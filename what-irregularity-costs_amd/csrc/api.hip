@@ -61,7 +61,7 @@ size_t voxel_bytes(int32_t storage) {
 struct OsnTsdfHipVolume {
     OsnTsdfHipConfig cfg{};
     bool reference_shape = false;  // created by osn_tsdf_hip_create
-    bool ok = false;
+    mutable bool ok = false;       // cleared by a launch failure or a device-side fault
     hipStream_t stream = nullptr;
     VolDev d{};
     uint32_t table_alloc = 0;      // slots allocated (live table + scratch region)
@@ -121,13 +121,11 @@ bool OsnTsdfHipVolume::alloc() {
     } else {
         HIP_OK(hipMalloc(&d.vox, n_vox * voxel_bytes(d.storage)));
     }
-    HIP_OK(hipMalloc(&d.new_keys, (size_t)cap * sizeof(int64_t)));
+    HIP_OK(hipMalloc(&d.new_keys, (size_t)cap * sizeof(TouchEntry)));
     HIP_OK(hipMalloc(&d.touched, (size_t)cap * sizeof(TouchEntry)));
-    HIP_OK(hipMalloc(&d.sort_a, (size_t)cap * sizeof(int64_t)));
-    HIP_OK(hipMalloc(&d.sort_b, (size_t)cap * sizeof(int64_t)));
-    const size_t n_flags = (size_t)kMaxSortLevels * sort_flag_stride(cap);
-    HIP_OK(hipMalloc(&d.sort_flags, n_flags * sizeof(int32_t)));
-    HIP_OK(hipMemsetAsync(d.sort_flags, 0, n_flags * sizeof(int32_t), stream));
+    HIP_OK(hipMalloc(&d.sort_a, (size_t)cap * sizeof(TouchEntry)));
+    HIP_OK(hipMalloc(&d.sort_b, (size_t)cap * sizeof(TouchEntry)));
+    HIP_OK(hipMalloc(&d.assign_done, 2 * (size_t)assign_done_per_parity(cap) * sizeof(int32_t)));
     if (reference_shape) {
         // the sentinel region past the live table (cpp/src/cuda_impl.cuh:61-67,
         // 84-88): keys -2 can never satisfy a CAS against the empty key, so
@@ -145,6 +143,8 @@ bool OsnTsdfHipVolume::reset() {
     // occupied at block (0,0,0)); counters and voxel storage are zeroed.
     HIP_OK(launch_fill_table(d.table, d.capacity, stream));
     HIP_OK(hipMemsetAsync(d.ctr, 0, sizeof(Counters), stream));
+    HIP_OK(hipMemsetAsync(d.assign_done, 0,
+                          2 * (size_t)assign_done_per_parity(d.capacity) * sizeof(int32_t), stream));
     const size_t n_vox = (size_t)cfg.pool_capacity_blocks * kBlockVoxels;
     if (d.storage == kStoreSums) {
         HIP_OK(hipMemsetAsync(d.tsdf, 0, n_vox * sizeof(float), stream));
@@ -169,7 +169,7 @@ OsnTsdfHipVolume::~OsnTsdfHipVolume() {
     }
     for (auto e : ev_pool) hipEventDestroy(e);
     void* ptrs[] = {d.table, d.ctr, d.block_coord, d.tsdf, d.weight, d.r, d.g, d.b, d.vox,
-                    d.new_keys, d.touched, d.sort_a, d.sort_b, d.sort_flags, ex_vcount, ex_voff,
+                    d.new_keys, d.touched, d.sort_a, d.sort_b, d.assign_done, ex_vcount, ex_voff,
                     ex_chunks, ex_flag_count, stage_posnor, stage_rgb};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -340,9 +340,23 @@ bool valid_frame(const float* d_depth, const OsnTsdfHipFrame* f) {
     return d_depth && f && f->width > 0 && f->height > 0 && f->fx != 0.0f && f->fy != 0.0f;
 }
 
+// Synchronise, then read one counter. The head of the counters (block_count,
+// fault) is read first: a fault means an index assignment abandoned a bounded
+// wait, so the pool order is incomplete and the volume is marked invalid.
 template <typename T>
 int32_t read_counter(const OsnTsdfHipVolume* v, size_t offset, T* out) {
     if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    int32_t head[2];
+    if (hipMemcpy(head, v->d.ctr, sizeof(head), hipMemcpyDeviceToHost) != hipSuccess) return 2;
+    if (head[1] != 0) {
+        std::fprintf(stderr, "[osn_tsdf_hip] index assignment gave up a wait; volume invalid\n");
+        v->ok = false;
+        return 2;
+    }
+    if (offset == offsetof(Counters, block_count) && sizeof(T) == sizeof(int32_t)) {
+        std::memcpy(out, head, sizeof(T));
+        return 0;
+    }
     if (hipMemcpy(out, reinterpret_cast<const char*>(v->d.ctr) + offset, sizeof(T),
                   hipMemcpyDeviceToHost) != hipSuccess)
         return 2;

@@ -58,13 +58,15 @@ struct alignas(128) ListCounters {
     int32_t pad0[31];
     int32_t touched_count;   // distinct slots the call visited
     int32_t pad1[31];
+    int32_t assign_ticket;   // next work item of the call's index assignment
+    int32_t pad2[31];
 };
 
 // Per-volume device counters, one allocation. block_count and drop_count are
 // what the reference device view exposes as int32* and uint64*.
 struct Counters {
     int32_t block_count;
-    int32_t stamp_unused;
+    uint32_t fault;                  // nonzero: a bounded wait of the assignment gave up
     unsigned long long drop_count;
     unsigned long long cas_attempts;
     unsigned long long lane_steps;   // sum over lanes of probe steps
@@ -112,12 +114,14 @@ struct VolDev {
     float weight_cap;
     int32_t steps;         // band half-width in samples
     int32_t w_max;         // projective weight saturation
-    // per-call work lists
-    int64_t* new_keys;     // capacity entries
-    struct TouchEntry* touched;  // capacity entries
-    int64_t* sort_a;       // capacity entries
-    int64_t* sort_b;       // capacity entries
-    int32_t* sort_flags;   // merge-tree arrival counters
+    // per-call work lists, capacity entries each; new_keys entries carry the
+    // slot the key was inserted at (bi unused), sort_a/sort_b are the merge
+    // buffers of the index assignment
+    TouchEntry* new_keys;
+    TouchEntry* touched;
+    TouchEntry* sort_a;
+    TouchEntry* sort_b;
+    int32_t* assign_done;  // per parity: finished work items per run and level
 };
 
 __host__ __device__ inline int64_t pack_coord(int32_t x, int32_t y, int32_t z) {

@@ -95,8 +95,10 @@ struct ExtractArgs {
     int32_t* count_out;    // device int32: vertex count or -1 on overflow
 };
 
+// Index assignment works in tiles of kSortTile new keys: level 0 sorts each
+// tile, level l >= 1 merges pairs of sorted runs of kSortTile << (l - 1) keys,
+// one kSortTile-output chunk per work item.
 constexpr int kSortTile = 4096;
-constexpr int kMaxSortLevels = 24;
 constexpr int kAssignMaxGrid = 256;
 
 hipError_t launch_alloc(const VolDev& v, const PointSrc& src, const AllocOpts& o,
@@ -117,8 +119,23 @@ hipError_t launch_fill_table(HashEntry* table, uint32_t n, hipStream_t s);
 hipError_t launch_cas_ceiling(unsigned long long* slots, uint32_t mask, int32_t n_ops,
                               hipStream_t s);
 
-__host__ __device__ inline int32_t sort_flag_stride(uint32_t capacity) {
-    return (int32_t)((capacity + kSortTile - 1) / kSortTile / 2 + 1);
+__host__ __device__ inline int32_t ceil_log2(int32_t x) {
+    int32_t l = 0;
+    while ((int32_t(1) << l) < x) ++l;
+    return l;
+}
+
+// A call inserts at most `capacity` keys, so at most this many tiles.
+__host__ __device__ inline int32_t assign_tiles_max(uint32_t capacity) {
+    return (int32_t)((capacity + kSortTile - 1) / kSortTile);
+}
+
+// assign_done holds, per parity, one counter per (level, run): level l of a
+// call with T tiles has ceil(T / 2^l) runs; the stride per level is the
+// volume's maximum tile count.
+__host__ __device__ inline int32_t assign_done_per_parity(uint32_t capacity) {
+    const int32_t t = assign_tiles_max(capacity);
+    return (ceil_log2(t) + 1) * t;
 }
 
 }  // namespace osn_hip
