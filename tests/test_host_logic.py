@@ -10,8 +10,9 @@ f32 = np.float32
 
 def band_walk_f32(p, cam, voxel, trunc, steps):
     """The reference band walk (cpp/src/volume.cpp:158-191) evaluated in numpy
-    float32, one rounded operation at a time: per sample, (culled, bound)
-    where bound is the device's shortcut bound."""
+    float32, one rounded operation at a time: per sample, (culled, B) with B
+    the device's shortcut distance, and the per-point threshold thr below
+    which the device skips the square root."""
     p = p.astype(f32)
     cam = cam.astype(f32)
     voxel = f32(voxel)
@@ -31,9 +32,10 @@ def band_walk_f32(p, cam, voxel, trunc, steps):
         e = c - cam
         sdf = dist - np.sqrt(e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1] + e[:, 2] * e[:, 2])
         culled = sdf < -trunc
-        bnd = np.where(dist + t >= 0, t, np.abs(t)) + f32(0.8661) * voxel + slack
-        out.append((culled, bnd))
-    return out, trunc
+        B = np.where(dist + t >= 0, t, np.abs(t))
+        out.append((culled, B))
+    thr = (trunc - f32(0.8661) * voxel) - slack
+    return out, thr
 
 
 @pytest.mark.parametrize("voxel,trunc,scale", [
@@ -41,8 +43,8 @@ def band_walk_f32(p, cam, voxel, trunc, steps):
     (0.01, 0.025, 3.0), (0.05, 0.11, 30.0),
 ])
 def test_cull_shortcut_never_hides_a_cull(voxel, trunc, scale):
-    """Whenever the device skips the square root (bound < trunc), the
-    reference expression would not have culled the sample either."""
+    """Whenever the device skips the square root (B < thr), the reference
+    expression would not have culled the sample either."""
     rng = np.random.default_rng(int(voxel * 1e4) + int(scale))
     n = 200000
     cam = rng.uniform(-scale, scale, 3)
@@ -52,10 +54,10 @@ def test_cull_shortcut_never_hides_a_cull(voxel, trunc, scale):
     r = np.concatenate([rng.uniform(0.0005, 2 * trunc, n // 4), rng.uniform(0.1, scale, n - n // 4)])
     p = cam + dirs * r[:, None]
     steps = max(1, math.ceil(np.float32(trunc) * (np.float32(1.0) / np.float32(voxel))))
-    samples, tr = band_walk_f32(p, cam, voxel, trunc, steps)
+    samples, thr = band_walk_f32(p, cam, voxel, trunc, steps)
     skipped = 0
-    for culled, bnd in samples:
-        skip = bnd < tr
+    for culled, B in samples:
+        skip = B < thr
         assert not np.any(culled & skip)
         skipped += int(skip.sum())
     assert skipped > 0

@@ -25,7 +25,7 @@
 namespace osn_hip {
 namespace {
 
-constexpr int kAllocThreads = 256;  // 4 waves; 16x16 pixels in depth mode
+constexpr int kAllocThreads = 256;  // 4 waves; an 8x8-pixel quarter tile each in depth mode
 constexpr int kKeysPerLane = 8;     // distinct keys a lane buffers before a flush
 constexpr int kWgList = 512;        // LDS touched-list capacity per workgroup
 constexpr int kWgNew = 256;         // LDS new-key-list capacity per workgroup
@@ -44,9 +44,23 @@ __device__ inline unsigned long long lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+}  // namespace
+}  // namespace osn_hip
+
+// Wave-wide reductions over the active lanes (DPP), from the device library
+// every HIP program links; the HIP headers declare the 64-bit forms only on
+// request.
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_add_u64(
+    unsigned long long);
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_or_u64(
+    unsigned long long);
+
+namespace osn_hip {
+namespace {
+
+// Wave-wide sum (every lane active).
 __device__ inline unsigned long long wave_sum_u64(unsigned long long x) {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    return x;
+    return __ockl_wfred_add_u64(x);
 }
 
 struct WgShared {
@@ -155,14 +169,22 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
     uint32_t slot = 0;
     uint32_t steps = 0;
     int32_t idx = kIdxPending;
+    int32_t stamp = 0;  // the slot's stamp as read with the key
     unsigned long long cas = 0, drops = 0;
     if (go) {
         slot = home_slot(v, key);
         for (uint32_t p = 0; p < v.probe_limit; ++p) {
             ++steps;
+            // key, index and stamp of the slot in one round trip: no index
+            // changes during this phase, and a key inserted during it still
+            // holds kIdxPending and stamp 0 from the empty slot
             const int64_t cur =
                 __hip_atomic_load(&v.table[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (cur == key) { found = true; break; }
+            const int32_t ci = __hip_atomic_load(&v.table[slot].block_idx, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            const int32_t cs = __hip_atomic_load(&v.table[slot].stamp, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == key) { found = true; idx = ci; stamp = cs; break; }
             if (cur == kEmptyKey) {
                 if (MODE == kAllocNoCas) break;
                 if (MODE == kAllocCountCas) cas += 1;
@@ -175,11 +197,7 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
             slot = next_slot(v, slot);
         }
         if (MODE != kAllocNoCas && !found && !inserted) drops += 1;  // probe bound hit
-        if (found) {
-            idx = __hip_atomic_load(&v.table[slot].block_idx, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-            if (idx == kIdxDropped) drops += 1;
-        }
+        if (found && idx == kIdxDropped) drops += 1;
     }
 
     // per-workgroup statistics (rare or measurement-only), one wave-level add
@@ -205,8 +223,11 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
         }
     }
 
+    // The first lane of the call to stamp the slot records the block as
+    // touched; a stamp already read as this call's means another workgroup
+    // did, and needs no exchange.
     bool touch = false;
-    if (TOUCH && (found || inserted)) {
+    if (TOUCH && (found || inserted) && stamp != c.stamp) {
         const int32_t old = atomicExch(&v.table[slot].stamp, c.stamp);
         touch = (old != c.stamp);
     }
@@ -227,10 +248,6 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
 // block below the first live key, the 64-bit marks are OR-ed across the wave,
 // and lane i takes window cell i. Only keys outside the window reach the
 // one-key-per-iteration loop.
-__device__ inline unsigned long long wave_or_u64(unsigned long long x) {
-    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
-    return x;
-}
 
 template <int MODE, bool TOUCH>
 __device__ void flush_wave(const TileCtx& c, LaneKeys& lk) {
@@ -257,7 +274,7 @@ __device__ void flush_wave(const TileCtx& c, LaneKeys& lk) {
                 lk.valid &= ~(1u << j);
             }
         }
-        cells = wave_or_u64(cells);
+        cells = __ockl_wfred_or_u64(cells);
         const int lane = __lane_id();
         const uint64_t mine = (uint64_t(ox + (lane & 3)) << (2 * kCoordBits)) |
                               (uint64_t(oy + ((lane >> 2) & 3)) << kCoordBits) |
@@ -282,77 +299,80 @@ __device__ void flush_wave(const TileCtx& c, LaneKeys& lk) {
     if (nd > 0) process_keys<MODE, TOUCH>(c, __lane_id() < nd, mine);
 }
 
-// One tile: 256 points (points / coords) or 16x16 pixels (depth), one per
-// thread. A depth pixel is back-projected in FP32 as
-//   xc = (u - cx) * z / fx, yc = (v - cy) * z / fy, p = R * (xc, yc, z) + cam
-// evaluated left to right without fused multiply-add.
-template <int MODE, bool TOUCH, int SRC>
-__device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t* coords, int tile) {
-    const VolDev& v = *c.v;
-    LaneKeys lk;
-    if (SRC == kSrcCoords) {
-        // Given keys have no spatial coherence to exploit within a wave, so
-        // each lane goes straight to the workgroup pre-hash with its key.
-        const int i = tile * kAllocThreads + threadIdx.x;
-        const bool has = i < src.n;
-        const int64_t key =
-            has ? pack_coord(coords[i * 3 + 0], coords[i * 3 + 1], coords[i * 3 + 2]) : kEmptyKey;
-        process_keys<MODE, TOUCH>(c, has, key);
-        return;
-    }
-    bool ok = false;
-    float px = 0.f, py = 0.f, pz = 0.f;
-    if (SRC == kSrcDepth) {
-        const int tiles_x = (src.W + 15) >> 4;
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int u = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-        const int w = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
-        if (u < src.W && w < src.H) {
-            const float z = src.depth[(size_t)w * src.W + u];
-            ok = (z > 0.0f) && __builtin_isfinite(z) && !(src.max_range > 0.0f && z > src.max_range);
-            if (ok) {
-                const float xc = ((float)u - src.cx) * z / src.fx;
-                const float yc = ((float)w - src.cy) * z / src.fy;
-                px = src.R[0] * xc + src.R[1] * yc + src.R[2] * z + src.cam[0];
-                py = src.R[3] * xc + src.R[4] * yc + src.R[5] * z + src.cam[1];
-                pz = src.R[6] * xc + src.R[7] * yc + src.R[8] * z + src.cam[2];
-            }
-        }
-    } else {
-        const int i = tile * kAllocThreads + threadIdx.x;
-        if (i < src.n) {
-            px = src.pos[i * 3 + 0];
-            py = src.pos[i * 3 + 1];
-            pz = src.pos[i * 3 + 2];
-            ok = true;
-        }
-    }
+// Band walk (volume.cpp:158-191) state of one point.
+struct Ray {
+    float px, py, pz;  // the point
+    float ux, uy, uz;  // unit direction from the camera
+    float dist;        // camera-to-point distance
+    float thr;         // samples with B < thr cannot be culled (see make_ray)
+};
 
-    // Band walk (volume.cpp:158-191).
-    float dist = 0.f, ux = 0.f, uy = 0.f, uz = 0.f, slack = 0.f;
-    if (ok) {
-        const float dx = px - src.cam[0], dy = py - src.cam[1], dz = pz - src.cam[2];
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        if (src.radius_sq > 0.0f && d2 > src.radius_sq) ok = false;
-        dist = sqrtf(d2);
-        if (!(dist > 1e-6f)) ok = false;
-        ux = dx / dist; uy = dy / dist; uz = dz / dist;
-        // Cull shortcut. The voxel centre chosen for a sample at signed offset
-        // t along the ray lies within half a voxel diagonal of the sample, so
-        // in exact arithmetic sdf >= -(B + 0.8661 voxel) with B = t when the
-        // sample is on the camera's side of the point's far end (dist + t >= 0)
-        // and |t| otherwise. Every FP32 rounding on the way is below 2^-23 of
-        // the magnitudes involved, a few dozen times over; `slack` bounds them
-        // all with a 64 * 2^-23 relative margin. A sample whose bound clears
-        // -trunc by more than the slack cannot be culled, so its square root
-        // is skipped; the others evaluate the reference's expression as is.
-        slack = 7.63e-6f * (fabsf(px) + fabsf(py) + fabsf(pz) + fabsf(src.cam[0]) +
-                            fabsf(src.cam[1]) + fabsf(src.cam[2]) + dist +
-                            (float)v.steps * v.voxel);
+// Sets the ray of point p up; false when the reference skips the point
+// (radius cull, dist <= 1e-6).
+__device__ inline bool make_ray(const VolDev& v, const PointSrc& src, float px, float py, float pz,
+                                Ray& r) {
+    const float dx = px - src.cam[0], dy = py - src.cam[1], dz = pz - src.cam[2];
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float dist = sqrtf(d2);
+    r.px = px; r.py = py; r.pz = pz;
+    r.dist = dist;
+    r.ux = dx / dist; r.uy = dy / dist; r.uz = dz / dist;
+    // Cull shortcut. The voxel centre chosen for a sample at signed offset t
+    // along the ray lies within half a voxel diagonal of the sample, so in
+    // exact arithmetic sdf >= -(B + 0.8661 voxel) with B = t when the sample
+    // is on the camera's side of the point's far end (dist + t >= 0) and |t|
+    // otherwise. Every FP32 rounding on the way is below 2^-23 of the
+    // magnitudes involved, a few dozen times over; `slack` bounds them all with
+    // a 64 * 2^-23 relative margin. A sample with B < thr cannot be culled, so
+    // its square root is skipped; the others evaluate the reference's
+    // expression as is.
+    const float slack = 7.63e-6f * (fabsf(px) + fabsf(py) + fabsf(pz) + fabsf(src.cam[0]) +
+                                    fabsf(src.cam[1]) + fabsf(src.cam[2]) + dist +
+                                    (float)v.steps * v.voxel);
+    r.thr = (v.trunc - 0.8661f * v.voxel) - slack;
+    return !(src.radius_sq > 0.0f && d2 > src.radius_sq) && (dist > 1e-6f);
+}
+
+// Sample s of a live lane's band walk: the voxel (p + u * t) / voxel, unless
+// culled, marks its block, in the wave's 4x4x4-block window (cells) or in the
+// lane buffer.
+__device__ inline void mark_sample(const VolDev& v, const PointSrc& src, const Ray& r, int s,
+                                   float t, int32_t ox, int32_t oy, int32_t oz,
+                                   unsigned long long& cells, LaneKeys& lk) {
+    const int32_t vx = (int32_t)floorf((r.px + r.ux * t) * v.inv_voxel);
+    const int32_t vy = (int32_t)floorf((r.py + r.uy * t) * v.inv_voxel);
+    const int32_t vz = (int32_t)floorf((r.pz + r.uz * t) * v.inv_voxel);
+    // t = s * voxel with voxel > 0, so B = t for s >= 0
+    const float B = s >= 0 ? t : (r.dist + t >= 0.0f ? t : -t);
+    bool culled = false;
+    if (B >= r.thr) {
+        const float cx = ((float)vx + 0.5f) * v.voxel;
+        const float cy = ((float)vy + 0.5f) * v.voxel;
+        const float cz = ((float)vz + 0.5f) * v.voxel;
+        const float ex = cx - src.cam[0], ey = cy - src.cam[1], ez = cz - src.cam[2];
+        const float sdf = r.dist - sqrtf(ex * ex + ey * ey + ez * ez);
+        culled = sdf < -v.trunc;
     }
-    // Keys inside a 4x4x4-block window anchored one block below the first live
-    // lane's point are marked in a 64-bit mask (one cell per lane later) with
-    // no buffering; only keys outside it go through the per-lane buffer.
+    if (!culled) {
+        const int32_t bx = floor_div8(vx), by = floor_div8(vy), bz = floor_div8(vz);
+        const uint32_t dx = uint32_t(bx - ox), dy = uint32_t(by - oy), dz = uint32_t(bz - oz);
+        if ((dx | dy | dz) < 4u) cells |= 1ull << (dx | (dy << 2) | (dz << 4));
+        else lk.push(pack_coord(bx, by, bz));
+    }
+}
+
+// Walks one point per lane (live lanes only) and hands every distinct block
+// of the wave to process_keys once. Keys inside a 4x4x4-block window anchored
+// one block below the first live lane's point are marked in a 64-bit mask
+// (lane i later takes cell i), with no buffering; only keys outside it go
+// through the lane buffer. STEPS is the band half-width in samples when known
+// at compile time (0: read it from the volume).
+template <int MODE, bool TOUCH, int STEPS>
+__device__ void walk_points(const TileCtx& c, const PointSrc& src, bool ok, float px, float py,
+                            float pz) {
+    const VolDev& v = *c.v;
+    Ray r{};
+    if (ok) ok = make_ray(v, src, px, py, pz, r);
     const unsigned long long live = __ballot(ok);
     if (live == 0) return;
     const int leader = __ffsll((long long)live) - 1;
@@ -360,36 +380,106 @@ __device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t*
     const int32_t oy = __builtin_amdgcn_readlane(floor_div8((int32_t)floorf(py * v.inv_voxel)), leader) - 1;
     const int32_t oz = __builtin_amdgcn_readlane(floor_div8((int32_t)floorf(pz * v.inv_voxel)), leader) - 1;
     unsigned long long cells = 0;
-    for (int32_t s = -v.steps; s <= v.steps; ++s) {
+    LaneKeys lk;
+    if (STEPS > 0) {
+        // A ray's 2 * STEPS + 1 samples span 2 * STEPS <= 8 voxels, so they
+        // meet at most 2 blocks per axis, 4 along the ray: the lane buffer
+        // (kKeysPerLane keys) never fills.
+        static_assert(STEPS <= 4 && kKeysPerLane >= 4, "lane buffer bound");
         if (ok) {
-            const float t = (float)s * v.voxel;
-            const int32_t vx = (int32_t)floorf((px + ux * t) * v.inv_voxel);
-            const int32_t vy = (int32_t)floorf((py + uy * t) * v.inv_voxel);
-            const int32_t vz = (int32_t)floorf((pz + uz * t) * v.inv_voxel);
-            const float bound = (dist + t >= 0.0f ? t : fabsf(t)) + 0.8661f * v.voxel + slack;
-            bool culled = false;
-            if (bound >= v.trunc) {
-                const float cx = ((float)vx + 0.5f) * v.voxel;
-                const float cy = ((float)vy + 0.5f) * v.voxel;
-                const float cz = ((float)vz + 0.5f) * v.voxel;
-                const float ex = cx - src.cam[0], ey = cy - src.cam[1], ez = cz - src.cam[2];
-                const float sdf = dist - sqrtf(ex * ex + ey * ey + ez * ez);
-                culled = sdf < -v.trunc;
-            }
-            if (!culled) {
-                const int32_t bx = floor_div8(vx), by = floor_div8(vy), bz = floor_div8(vz);
-                const uint32_t dx = uint32_t(bx - ox), dy = uint32_t(by - oy), dz = uint32_t(bz - oz);
-                if ((dx | dy | dz) < 4u) cells |= 1ull << (dx | (dy << 2) | (dz << 4));
-                else lk.push(pack_coord(bx, by, bz));
-            }
+#pragma unroll
+            for (int s = -STEPS; s <= STEPS; ++s)
+                mark_sample(v, src, r, s, (float)s * v.voxel, ox, oy, oz, cells, lk);
         }
-        if (__ballot(lk.valid >> (kKeysPerLane - 1))) flush_wave<MODE, TOUCH>(c, lk);
+    } else {
+        for (int32_t s = -v.steps; s <= v.steps; ++s) {
+            if (ok) mark_sample(v, src, r, s, (float)s * v.voxel, ox, oy, oz, cells, lk);
+            if (__ballot(lk.valid >> (kKeysPerLane - 1))) flush_wave<MODE, TOUCH>(c, lk);
+        }
     }
-    cells = wave_or_u64(cells);
+    cells = __ockl_wfred_or_u64(cells);
     const int lane = __lane_id();
     process_keys<MODE, TOUCH>(c, (cells >> lane) & 1ull,
                               pack_coord(ox + (lane & 3), oy + ((lane >> 2) & 3), oz + (lane >> 4)));
     flush_wave<MODE, TOUCH>(c, lk);
+}
+
+template <int MODE, bool TOUCH>
+__device__ inline void walk_points_any(const TileCtx& c, const PointSrc& src, bool ok, float px,
+                                       float py, float pz) {
+    if (c.v->steps == 4) walk_points<MODE, TOUCH, 4>(c, src, ok, px, py, pz);
+    else walk_points<MODE, TOUCH, 0>(c, src, ok, px, py, pz);
+}
+
+// Depth of pixel (u, w), 0 (invalid) outside the image.
+__device__ inline float load_depth(const PointSrc& src, int u, int w) {
+    return (u < src.W && w < src.H) ? src.depth[(size_t)w * src.W + u] : 0.0f;
+}
+
+// A depth pixel is back-projected in FP32 as
+//   xc = (u - cx) * z / fx, yc = (v - cy) * z / fy, p = R * (xc, yc, z) + cam
+// evaluated left to right without fused multiply-add; false for a pixel the
+// walk skips (z <= 0, not finite, or beyond max_range when that is set).
+__device__ inline bool backproject_pixel(const PointSrc& src, int u, int w, float z, float& px,
+                                         float& py, float& pz) {
+    const bool ok =
+        (z > 0.0f) && __builtin_isfinite(z) && !(src.max_range > 0.0f && z > src.max_range);
+    if (ok) {
+        const float xc = ((float)u - src.cx) * z / src.fx;
+        const float yc = ((float)w - src.cy) * z / src.fy;
+        px = src.R[0] * xc + src.R[1] * yc + src.R[2] * z + src.cam[0];
+        py = src.R[3] * xc + src.R[4] * yc + src.R[5] * z + src.cam[1];
+        pz = src.R[6] * xc + src.R[7] * yc + src.R[8] * z + src.cam[2];
+    }
+    return ok;
+}
+
+// Depth images are walked in work units of sub x sub tiles of 16x16 pixels
+// (sub = 1 for a single frame, whose 16x16 tiles already fill the chip; 2 in
+// batches): the workgroup keeps its pre-hash across the unit's tiles, each of
+// its four waves takes an 8x8 quarter of a tile, and the depth of the next
+// tile is loaded before the current one is walked.
+template <int MODE, bool TOUCH>
+__device__ void alloc_depth_unit(const TileCtx& c, const PointSrc& src, int unit, int sub) {
+    const int span = 16 * sub;
+    const int per_row = (src.W + span - 1) / span;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int u0 = (unit % per_row) * span + (wave & 1) * 8 + (lane & 7);
+    const int w0 = (unit / per_row) * span + (wave >> 1) * 8 + (lane >> 3);
+    const int n = sub * sub;
+    float z = load_depth(src, u0, w0);
+#pragma unroll 1
+    for (int j = 0; j < n; ++j) {
+        const int u = u0 + (j % sub) * 16, w = w0 + (j / sub) * 16;
+        const int jn = j + 1;
+        const float zn = jn < n ? load_depth(src, u0 + (jn % sub) * 16, w0 + (jn / sub) * 16) : 0.0f;
+        float px = 0.f, py = 0.f, pz = 0.f;
+        const bool ok = backproject_pixel(src, u, w, z, px, py, pz);
+        walk_points_any<MODE, TOUCH>(c, src, ok, px, py, pz);
+        z = zn;
+    }
+}
+
+// One tile of 256 points (points / coords), one per thread.
+template <int MODE, bool TOUCH, int SRC>
+__device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t* coords, int tile) {
+    const int i = tile * kAllocThreads + threadIdx.x;
+    const bool has = i < src.n;
+    if (SRC == kSrcCoords) {
+        // Given keys have no spatial coherence to exploit within a wave, so
+        // each lane goes straight to the workgroup pre-hash with its key.
+        const int64_t key =
+            has ? pack_coord(coords[i * 3 + 0], coords[i * 3 + 1], coords[i * 3 + 2]) : kEmptyKey;
+        process_keys<MODE, TOUCH>(c, has, key);
+        return;
+    }
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (has) {
+        px = src.pos[i * 3 + 0];
+        py = src.pos[i * 3 + 1];
+        pz = src.pos[i * 3 + 2];
+    }
+    walk_points_any<MODE, TOUCH>(c, src, has, px, py, pz);
 }
 
 __device__ void wg_init(WgShared& s, int set_size) {
@@ -450,29 +540,33 @@ __global__ __launch_bounds__(kAllocThreads) void k_alloc(VolDev v, PointSrc src,
     const TileCtx c{&v, &v.ctr->lists[call.parity], &s, set_size, call.stamp};
     for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         if (tile != (int)blockIdx.x) clear_set(s, set_size);
-        alloc_tile<MODE, TOUCH, SRC>(c, src, coords, tile);
+        if (SRC == kSrcDepth) alloc_depth_unit<MODE, TOUCH>(c, src, tile, 1);
+        else alloc_tile<MODE, TOUCH, SRC>(c, src, coords, tile);
     }
     flush_lists(v, c.lc, s);
     flush_stats<MODE>(v, s);
 }
 
-// Many scenes in one launch: tiles of all active scenes form one work list;
-// each scene keeps its own table and lists, flushed when the workgroup moves
-// on to another scene.
+// Many scenes in one launch: the depth work units of all active scenes form
+// one work list (scene s owns [tile_prefix[s], tile_prefix[s + 1]), read once
+// into LDS); each scene keeps its own table and lists, flushed when the
+// workgroup moves on to another scene.
 __global__ __launch_bounds__(kAllocThreads) void k_alloc_batch(BatchDev b, int32_t set_size,
                                                                int32_t total_tiles) {
     __shared__ WgShared s;
+    __shared__ int32_t s_prefix[kMaxBatchScenes + 1];
     wg_init(s, set_size);
+    for (int i = threadIdx.x; i <= b.n_scenes; i += blockDim.x) s_prefix[i] = b.tile_prefix[i];
     __syncthreads();
     const int k = b.step_ctr ? *b.step_ctr : b.step;
     int cur = -1;
+    int sc = 0;
     CallInfo call{0, 0};
     VolDev v{};
     for (int tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
-        int sc = 0;
-        while (sc + 1 < b.n_scenes && b.tile_prefix[sc + 1] <= tile) ++sc;
+        while (sc + 1 < b.n_scenes && s_prefix[sc + 1] <= tile) ++sc;  // tiles ascend
         const SceneStep& st = b.steps[k * b.n_scenes + sc];
-        const int local = tile - b.tile_prefix[sc];
+        const int local = tile - s_prefix[sc];
         if (local == 0 && threadIdx.x == 0)
             reset_other_parity(b.vols[sc].ctr, batch_call(b, sc, k).parity);
         if (!st.active || local >= st.tiles) continue;
@@ -487,7 +581,7 @@ __global__ __launch_bounds__(kAllocThreads) void k_alloc_batch(BatchDev b, int32
         }
         clear_set(s, set_size);
         const TileCtx c{&v, &v.ctr->lists[call.parity], &s, set_size, call.stamp};
-        alloc_tile<kAllocNormal, true, kSrcDepth>(c, st.src, nullptr, local);
+        alloc_depth_unit<kAllocNormal, true>(c, st.src, local, kBatchDepthSub);
     }
     if (cur >= 0) {
         flush_lists(v, &v.ctr->lists[call.parity], s);
@@ -752,7 +846,17 @@ __device__ bool assign_item(const VolDev& v, const AssignCall& a, AssignLds& s, 
         drops = wave_sum_u64(drops);
         if (__lane_id() == 0 && drops) atomicAdd(&s.drops, drops);
     }
-    const int old = publish_count(a.done + level * a.t_stride + (c >> level));
+    int32_t* mine = a.done + level * a.t_stride + (c >> level);
+    int old = 0;
+    if (!top) {
+        old = publish_count(mine);
+    } else {
+        // nothing in this kernel reads a top item's stores: count it, no fence;
+        // a one-tile call has nothing to count
+        __syncthreads();
+        if (threadIdx.x == 0 && a.T > 1)
+            old = __hip_atomic_fetch_add(mine, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (threadIdx.x == 0) {
         if (top && s.drops) atomicAdd(&v.ctr->drop_count, s.drops);
         s.drops = 0;
@@ -883,7 +987,7 @@ template <int MODE, bool TOUCH>
 static void launch_alloc_t(const VolDev& v, const PointSrc& src, const int32_t* coords, int set,
                            CallInfo call, int grid_cap, hipStream_t s) {
     int tiles;
-    if (src.depth && !coords) tiles = ((src.W + 15) / 16) * ((src.H + 15) / 16);
+    if (src.depth && !coords) tiles = depth_units(src.W, src.H, 1);
     else tiles = (src.n + kAllocThreads - 1) / kAllocThreads;
     const int grid = grid_cap > 0 ? min(tiles, grid_cap) : tiles;
     if (coords)

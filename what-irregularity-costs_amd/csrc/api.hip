@@ -873,7 +873,7 @@ struct OsnTsdfHipBatch {
 extern "C" {
 
 OsnTsdfHipBatch* osn_tsdf_hip_batch_create(OsnTsdfHipVolume* const* volumes, int32_t n) {
-    if (!volumes || n <= 0 || n > 64) return nullptr;
+    if (!volumes || n <= 0 || n > kMaxBatchScenes) return nullptr;
     for (int i = 0; i < n; ++i) {
         if (!volumes[i] || !volumes[i]->ok || volumes[i]->in_batch) return nullptr;
         if (volumes[i]->d.storage == kStoreSums) return nullptr;
@@ -924,7 +924,7 @@ int32_t osn_tsdf_hip_batch_set_schedule(OsnTsdfHipBatch* b, int32_t n_steps,
             st.active = 1;
             st.src = depth_src(d_depths[i], &frames[i]);
             st.cam = proj_cam(d_depths[i], &frames[i]);
-            st.tiles = ((frames[i].width + 15) / 16) * ((frames[i].height + 15) / 16);
+            st.tiles = depth_units(frames[i].width, frames[i].height, kBatchDepthSub);
             if (st.tiles > tiles[s]) tiles[s] = st.tiles;
         }
     std::vector<int32_t> prefix(n + 1, 0);

@@ -58,8 +58,18 @@ struct SceneStep {
     PointSrc src;
     ProjCam cam;
     int32_t active;
-    int32_t tiles;      // 16x16 pixel tiles of this scene's image
+    int32_t tiles;      // depth work units of this scene's image (kBatchDepthSub)
 };
+
+// Depth images are allocated in work units of sub x sub tiles of 16x16
+// pixels, one workgroup each; batches use kBatchDepthSub.
+constexpr int kBatchDepthSub = 2;
+__host__ __device__ inline int32_t depth_units(int32_t W, int32_t H, int32_t sub) {
+    const int32_t span = 16 * sub;
+    return ((W + span - 1) / span) * ((H + span - 1) / span);
+}
+
+constexpr int kMaxBatchScenes = 64;  // volumes per batch
 
 // A batched step over many scenes, read from device memory so that a captured
 // graph can replay it: scene s of step k is steps[k * n_scenes + s], with k

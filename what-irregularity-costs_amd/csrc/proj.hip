@@ -241,7 +241,6 @@ __global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, Call
 // list is cut into one contiguous chunk per XCD (workgroups are dealt to the
 // 8 XCDs round-robin by id), so each XCD's L2 serves the depth images of only
 // the few scenes in its chunk.
-constexpr int kMaxBatchScenes = 64;
 constexpr int kXcds = 8;
 
 template <int STORE>
