@@ -653,8 +653,8 @@ int32_t orc_backproject_f32(const float* depth, int32_t W, int32_t H, float fx, 
  * band walk visits), with the camera transform R_cw (world -> camera) and the
  * camera centre c:
  *   p = R_cw * ((v + 0.5) * voxel - c)
- *   skip unless p.z > 0; iz = 1 / p.z
- *   pixel = (floor(fx * (p.x * iz) + cx + 0.5), floor(fy * (p.y * iz) + cy + 0.5))
+ *   skip unless p.z > 0
+ *   pixel = (floor(fx * p.x / p.z + cx + 0.5), floor(fy * p.y / p.z + cy + 0.5))
  *   skip if outside the image, or if its depth d is not > 0
  *   sdf = d - p.z; skip if sdf < -trunc; x = min(sdf * (1 / trunc), 1)
  *   t <- (t * w + x) / (w + 1);  w <- min(w + 1, w_max)
@@ -665,11 +665,20 @@ int32_t orc_backproject_f32(const float* depth, int32_t W, int32_t H, float fx, 
  * depth discontinuity measures which side of a pixel edge a voxel projected
  * to, not storage or accumulation precision.
  *
- * The FP32 part is the device's exact expression sequence:
- *   o  = (float)(b * 8) * voxel + 0.5f * voxel - c        (block origin centre)
- *   d0 = R_cw * o                                         (row dot, left to right)
- *   row = (d0 + ly * ey) + lz * ez,  e* = voxel * R_cw[:, *]
- *   p   = row + lx * ex                   (one row per (ly, lz), shared along x)
+ * The FP32 part is the device's exact expression sequence, in which the
+ * pixel coordinates are ratios of two linear forms of the voxel index:
+ *   A = fx * p.x + (cx + 0.5) * p.z,  B = fy * p.y + (cy + 0.5) * p.z,  Z = p.z
+ *   per frame:  e_j = voxel * R_cw[:, j]        (j = x, y, z voxel axes)
+ *               (A_j, B_j, Z_j) = (fx * e_j.x + cxh * e_j.z, fy * e_j.y + cyh * e_j.z, e_j.z)
+ *               cxh = cx + 0.5, cyh = cy + 0.5
+ *   per block:  o  = (float)(b * 8) * voxel + 0.5f * voxel - c   (block origin centre)
+ *               d0 = R_cw * o                   (row dot, left to right)
+ *               (A0, B0, Z0) = (fx * d0.x + cxh * d0.z, fy * d0.y + cyh * d0.z, d0.z)
+ *   per run of 4 voxels along x (lx0 = 0 or 4):
+ *               off = (ly * F_y + lz * F_z) + lx0 * F_x      (F = A, B, Z)
+ *               F(lx0) = F0 + off,  F(lx + 1) = F(lx) + F_x
+ *   per voxel:  skip unless Z > 0; iz = 1 / Z; uf = A * iz; vf = B * iz
+ *               skip unless 0 <= uf < W and 0 <= vf < H; pixel = ((int)uf, (int)vf)
  * with each product rounded separately (no FMA).
  */
 typedef struct {
@@ -691,30 +700,43 @@ typedef struct {
 static ProjDecision proj_decide(const OrcProjCam* k, const float* depth, float voxel, float trunc,
                                 int32_t bx, int32_t by, int32_t bz, int lx, int ly, int lz) {
     ProjDecision r = {0, 0, 0, 0.0f, 0.0f};
+    const float cxh = k->cx + 0.5f, cyh = k->cy + 0.5f;
+    float g[3][3];  /* g[F][j]: F = A, B, Z; j = voxel axis */
+    for (int j = 0; j < 3; ++j) {
+        const float ex = voxel * k->Rcw[0 * 3 + j];
+        const float ey = voxel * k->Rcw[1 * 3 + j];
+        const float ez = voxel * k->Rcw[2 * 3 + j];
+        g[0][j] = k->fx * ex + cxh * ez;
+        g[1][j] = k->fy * ey + cyh * ez;
+        g[2][j] = ez;
+    }
     const float h = 0.5f * voxel;
     const float ox = (float)(bx * 8) * voxel + h - k->c[0];
     const float oy = (float)(by * 8) * voxel + h - k->c[1];
     const float oz = (float)(bz * 8) * voxel + h - k->c[2];
-    float d0[3], e[3][3];
-    for (int i = 0; i < 3; ++i) {
-        d0[i] = k->Rcw[i * 3 + 0] * ox + k->Rcw[i * 3 + 1] * oy + k->Rcw[i * 3 + 2] * oz;
-        for (int j = 0; j < 3; ++j) e[j][i] = voxel * k->Rcw[i * 3 + j];
-    }
-    const float fl[3] = {(float)lx, (float)ly, (float)lz};
-    float p[3];
+    float d0[3];
     for (int i = 0; i < 3; ++i)
-        p[i] = (d0[i] + fl[1] * e[1][i] + fl[2] * e[2][i]) + fl[0] * e[0][i];
-    if (!(p[2] > 0.0f)) return r;
-    const float iz = 1.0f / p[2];
-    const float uf = k->fx * (p[0] * iz) + k->cx + 0.5f;
-    const float vf = k->fy * (p[1] * iz) + k->cy + 0.5f;
+        d0[i] = k->Rcw[i * 3 + 0] * ox + k->Rcw[i * 3 + 1] * oy + k->Rcw[i * 3 + 2] * oz;
+    const float f0[3] = {k->fx * d0[0] + cxh * d0[2], k->fy * d0[1] + cyh * d0[2], d0[2]};
+    const int lx0 = lx & 4;
+    float f[3];
+    for (int q = 0; q < 3; ++q) {
+        const float off = ((float)ly * g[q][1] + (float)lz * g[q][2]) + (float)lx0 * g[q][0];
+        f[q] = f0[q] + off;
+        for (int j = lx0; j < lx; ++j) f[q] = f[q] + g[q][0];
+    }
+    const float Z = f[2];
+    if (!(Z > 0.0f)) return r;
+    const float iz = 1.0f / Z;
+    const float uf = f[0] * iz;
+    const float vf = f[1] * iz;
     if (!(uf >= 0.0f) || !(vf >= 0.0f) || !(uf < (float)k->W) || !(vf < (float)k->H)) return r;
-    const int32_t u = (int32_t)floorf(uf), v = (int32_t)floorf(vf);
+    const int32_t u = (int32_t)uf, v = (int32_t)vf;
     const float d = depth[(size_t)v * k->W + u];
     if (!(d > 0.0f)) return r;
-    const float sdf = d - p[2];
+    const float sdf = d - Z;
     if (sdf < -trunc) return r;
-    r.valid = 1; r.u = u; r.v = v; r.z32 = p[2]; r.sdf32 = sdf;
+    r.valid = 1; r.u = u; r.v = v; r.z32 = Z; r.sdf32 = sdf;
     return r;
 }
 
