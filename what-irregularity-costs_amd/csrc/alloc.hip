@@ -907,7 +907,15 @@ __global__ __launch_bounds__(1024) void k_assign_batch(BatchDev b) {
     __shared__ AssignLds s;
     const int k = b.step_ctr ? *b.step_ctr : b.step;
     const int sc = blockIdx.y;
-    if (!b.steps[k * b.n_scenes + sc].active) return;
+    const SceneStep& st = b.steps[k * b.n_scenes + sc];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the scene's update parameters for this step, built once here (the
+        // touched list is final) rather than by every update workgroup
+        UpdParams p = make_params(b.vols[sc], st.cam, batch_call(b, sc, k).parity);
+        if (!st.active) p.n = 0;
+        b.upd[sc] = p;
+    }
+    if (!st.active) return;
     const VolDev v = b.vols[sc];
     assign_volume(v, batch_call(b, sc, k).parity, blockIdx.x, s);
 }

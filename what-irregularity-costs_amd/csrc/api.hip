@@ -806,6 +806,7 @@ struct OsnTsdfHipBatch {
     int32_t* d_tile_prefix = nullptr;
     int32_t* d_step = nullptr;
     SceneStep* d_steps = nullptr;
+    UpdParams* d_upd = nullptr;
     int32_t n_steps = 0;
     int32_t cursor = 0;
     int32_t total_tiles = 0;
@@ -832,6 +833,7 @@ struct OsnTsdfHipBatch {
         b.tile_prefix = d_tile_prefix;
         b.step_ctr = from_device ? d_step : nullptr;
         b.call_base = d_call_base;
+        b.upd = d_upd;
         b.step = step;
         b.n_scenes = (int32_t)vols.size();
         return b;
@@ -863,7 +865,7 @@ struct OsnTsdfHipBatch {
         drop_graph();
         for (auto& p : ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
         for (void* p : {(void*)d_vols, (void*)d_call_base, (void*)d_tile_prefix, (void*)d_step,
-                        (void*)d_steps})
+                        (void*)d_steps, (void*)d_upd})
             if (p) hipFree(p);
         if (stream) hipStreamDestroy(stream);
         for (auto* v : vols) v->in_batch = false;
@@ -897,6 +899,7 @@ OsnTsdfHipBatch* osn_tsdf_hip_batch_create(OsnTsdfHipVolume* const* volumes, int
               hipMalloc(&b->d_call_base, sizeof(int32_t) * n) == hipSuccess &&
               hipMalloc(&b->d_tile_prefix, sizeof(int32_t) * (n + 1)) == hipSuccess &&
               hipMalloc(&b->d_step, sizeof(int32_t)) == hipSuccess &&
+              hipMalloc(&b->d_upd, sizeof(UpdParams) * n) == hipSuccess &&
               hipMemcpy(b->d_vols, hv.data(), sizeof(VolDev) * n, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) { delete b; return nullptr; }
     for (auto* v : b->vols) v->in_batch = true;

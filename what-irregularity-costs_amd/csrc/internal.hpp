@@ -1,7 +1,7 @@
 // Host-side launch interface between the C ABI layer and the kernels.
 #pragma once
 
-#include "common.hpp"
+#include "upd_params.hpp"
 
 namespace osn_hip {
 
@@ -17,18 +17,6 @@ struct PointSrc {
     float max_range;       // drop pixels with depth > max_range (> 0 enables)
     float cam[3];          // camera centre in world (the band walk's origin)
     float radius_sq;       // band-walk far cull, 0 disables
-};
-
-// Camera for the projective update: world -> camera rotation and the camera
-// centre, FP32, plus intrinsics and the depth image.
-struct ProjCam {
-    float Rcw[9];
-    float c[3];
-    float fx, fy, cx, cy;
-    int32_t W, H;
-    const float* depth;
-    double Rcw64[9];   // FP64 pose, read only by the FP64 oracle kernel
-    double c64[3];
 };
 
 // Which call this is: the stamp marks blocks touched by it, the parity picks
@@ -80,6 +68,7 @@ struct BatchDev {
     const int32_t* tile_prefix;  // n_scenes + 1, exclusive prefix of tiles
     const int32_t* step_ctr;
     const int32_t* call_base;    // per scene: calls the volume had made before step 0
+    UpdParams* upd;              // per scene: the update's parameters, written each step
     int32_t step;
     int32_t n_scenes;
 };

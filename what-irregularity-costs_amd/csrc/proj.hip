@@ -39,62 +39,19 @@
 namespace osn_hip {
 namespace {
 
-// Everything the update reads for one scene, gathered once per launch (kernel
-// arguments in the single-volume kernel, LDS in the batched one) so that the
-// per-voxel loop never re-reads descriptors from global memory.
+// Pointers of the parameters are dereferenced through the global address
+// space, so every access is a global (not flat) load or store; a batch reads
+// its per-scene parameter array through the constant address space (scalar
+// loads).
 #define OSN_GLOBAL __attribute__((address_space(1)))
+#define OSN_CONST __attribute__((address_space(4)))
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
-struct UpdParams {
-    // global address space, so loads through pointers kept in LDS stay
-    // global (not flat) loads
-    const HashEntry OSN_GLOBAL* table;
-    const int32_t OSN_GLOBAL* block_coord;
-    char OSN_GLOBAL* vox;
-    const TouchEntry OSN_GLOBAL* touched;
-    const float OSN_GLOBAL* depth;
-    float Rcw[9];
-    float c[3];
-    float g[3][3];      // g[F][j]: step of form F (A, B, Z) along voxel axis j
-    float fx, fy, cxh, cyh;
-    int32_t W, H;
-    float voxel, trunc, inv_trunc, wmax;
-    int32_t n;          // touched blocks
-    int32_t pad;
-    double Rcw64[9];    // FP64 oracle kernel only
-    double c64[3];
-};
-
-__device__ inline UpdParams make_params(const VolDev& v, const ProjCam& k, int parity) {
-    UpdParams p;
-    p.table = (const HashEntry OSN_GLOBAL*)v.table;
-    p.block_coord = (const int32_t OSN_GLOBAL*)v.block_coord;
-    p.vox = (char OSN_GLOBAL*)v.vox;
-    p.touched = (const TouchEntry OSN_GLOBAL*)v.touched;
-    p.depth = (const float OSN_GLOBAL*)k.depth;
-    for (int i = 0; i < 9; ++i) { p.Rcw[i] = k.Rcw[i]; p.Rcw64[i] = k.Rcw64[i]; }
-    for (int i = 0; i < 3; ++i) { p.c[i] = k.c[i]; p.c64[i] = k.c64[i]; }
-    p.fx = k.fx; p.fy = k.fy;
-    p.cxh = k.cx + 0.5f;
-    p.cyh = k.cy + 0.5f;
-    for (int j = 0; j < 3; ++j) {
-        const float ex = v.voxel * k.Rcw[0 * 3 + j];
-        const float ey = v.voxel * k.Rcw[1 * 3 + j];
-        const float ez = v.voxel * k.Rcw[2 * 3 + j];
-        p.g[0][j] = p.fx * ex + p.cxh * ez;
-        p.g[1][j] = p.fy * ey + p.cyh * ez;
-        p.g[2][j] = ez;
-    }
-    p.W = k.W; p.H = k.H;
-    p.voxel = v.voxel;
-    p.trunc = v.trunc;
-    p.inv_trunc = 1.0f / v.trunc;
-    p.wmax = (float)v.w_max;
-    p.n = v.ctr->lists[parity].touched_count;
-    p.pad = 0;
-    return p;
+template <typename T>
+__device__ inline T OSN_GLOBAL* glob(T* p) {
+    return (T OSN_GLOBAL*)p;
 }
 
 // A lane's two runs: voxels lane * 4 + 256 * r + [0, 4) of the block, r = 0, 1.
@@ -103,7 +60,8 @@ struct LaneRuns {
     int32_t lx0, ly, lz0;
 };
 
-__device__ inline LaneRuns lane_runs(const UpdParams& k, int lane) {
+template <typename Params>  // UpdParams in any address space
+__device__ inline LaneRuns lane_runs(const Params& k, int lane) {
     LaneRuns L;
     const int vi = lane * 4;
     L.lx0 = vi & 7;
@@ -119,7 +77,8 @@ __device__ inline LaneRuns lane_runs(const UpdParams& k, int lane) {
 }
 
 // Block-origin values (A0, B0, Z0) of block (bx, by, bz).
-__device__ inline void block_forms(const UpdParams& k, int32_t bx, int32_t by, int32_t bz,
+template <typename Params>
+__device__ inline void block_forms(const Params& k, int32_t bx, int32_t by, int32_t bz,
                                    float f0[3]) {
     const float voxel = k.voxel;
     const float h = 0.5f * voxel;
@@ -145,8 +104,9 @@ __device__ inline float avg_step(float t, float w, float x) {
 // lane (lanes 0-3), so the value stays a vector register while the load is in
 // flight: the next block's entry is requested a whole block ahead, and the
 // wave waits for it only when it reads the dwords out (touch_entry).
-__device__ inline uint32_t load_touch(const UpdParams& k, int item, int lane) {
-    return reinterpret_cast<const uint32_t OSN_GLOBAL*>(k.touched + item)[lane & 3];
+template <typename Params>
+__device__ inline uint32_t load_touch(const Params& k, int item, int lane) {
+    return reinterpret_cast<const uint32_t OSN_GLOBAL*>(glob(k.touched + item))[lane & 3];
 }
 
 __device__ inline u32x4 touch_entry(uint32_t part) {
@@ -161,11 +121,11 @@ __device__ inline u32x4 touch_entry(uint32_t part) {
 // block needs from memory is requested before any of it is used: the lane's
 // two voxel records, then the depth of all 8 of its voxels (pixels chosen
 // without branches: an out-of-image voxel reads pixel 0 and is then rejected).
-template <int STORE>
-__device__ inline void update_block(const UpdParams& k, const LaneRuns& L, uint32_t part, int lane) {
+template <int STORE, typename Params>
+__device__ inline void update_block(const Params& k, const LaneRuns& L, uint32_t part, int lane) {
     const u32x4 e = touch_entry(part);
     int32_t bi = (int32_t)e.w;
-    if (bi == kIdxPending) bi = k.table[e.z].block_idx;  // inserted by this call
+    if (bi == kIdxPending) bi = glob(k.table)[e.z].block_idx;  // inserted by this call
     if (bi < 0) return;
     int32_t bx, by, bz;
     unpack_key((int64_t)(((uint64_t)e.y << 32) | e.x), bx, by, bz);
@@ -174,11 +134,11 @@ __device__ inline void update_block(const UpdParams& k, const LaneRuns& L, uint3
     u32x4 h16[2];
     f32x4 h32[2][2];
     if constexpr (STORE == kStoreF16) {
-        const u32x4 OSN_GLOBAL* rec = reinterpret_cast<const u32x4 OSN_GLOBAL*>(k.vox + v0 * 4);
+        const u32x4 OSN_GLOBAL* rec = reinterpret_cast<const u32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 4);
         h16[0] = rec[0];
         h16[1] = rec[64];  // run 1 starts 256 voxels (1 KiB) later
     } else if constexpr (STORE == kStoreF32) {
-        const f32x4 OSN_GLOBAL* rec = reinterpret_cast<const f32x4 OSN_GLOBAL*>(k.vox + v0 * 8);
+        const f32x4 OSN_GLOBAL* rec = reinterpret_cast<const f32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 8);
         h32[0][0] = rec[0];
         h32[0][1] = rec[1];
         h32[1][0] = rec[128];
@@ -212,7 +172,7 @@ __device__ inline void update_block(const UpdParams& k, const LaneRuns& L, uint3
     }
     float d[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) d[i] = k.depth[pix[i]];
+    for (int i = 0; i < 8; ++i) d[i] = glob(k.depth)[pix[i]];
     // The records are read before the gates decide whether they change; left
     // alone, the compiler would move a record load behind the gates (into the
     // store branch) and expose its latency. This empty statement, which emits
@@ -246,7 +206,7 @@ __device__ inline void update_block(const UpdParams& k, const LaneRuns& L, uint3
                         (uint32_t)wn, (uint32_t)__half_as_ushort(__float2half_rn(tn)), 0x05040100u);
                     word[j] = o[j] ? nw : word[j];
                 }
-                u32x4 OSN_GLOBAL* rec = reinterpret_cast<u32x4 OSN_GLOBAL*>(k.vox + v0 * 4);
+                u32x4 OSN_GLOBAL* rec = reinterpret_cast<u32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 4);
                 rec[r * 64] = u32x4{word[0], word[1], word[2], word[3]};
             }
         } else if constexpr (STORE == kStoreF32) {
@@ -261,7 +221,7 @@ __device__ inline void update_block(const UpdParams& k, const LaneRuns& L, uint3
                     tv[j] = o[j] ? tn : tv[j];
                     wv[j] = o[j] ? wn : wv[j];
                 }
-                f32x4 OSN_GLOBAL* rec = reinterpret_cast<f32x4 OSN_GLOBAL*>(k.vox + v0 * 8);
+                f32x4 OSN_GLOBAL* rec = reinterpret_cast<f32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 8);
                 if (o[0] | o[1]) rec[r * 128 + 0] = f32x4{tv[0], wv[0], tv[1], wv[1]};
                 if (o[2] | o[3]) rec[r * 128 + 1] = f32x4{tv[2], wv[2], tv[3], wv[3]};
             }
@@ -271,7 +231,7 @@ __device__ inline void update_block(const UpdParams& k, const LaneRuns& L, uint3
             for (int j = 0; j < 4; ++j) {
                 if (!o[j]) continue;
                 f64x2 OSN_GLOBAL* rec = reinterpret_cast<f64x2 OSN_GLOBAL*>(
-                    k.vox + (v0 + r * 256 + j) * 16);
+                    glob(k.vox) + (v0 + r * 256 + j) * 16);
                 f64x2 q = *rec;
                 const double vs = (double)k.voxel;
                 const double wx = ((double)(bx * 8 + L.lx0 + j) + 0.5) * vs - k.c64[0];
@@ -319,22 +279,14 @@ constexpr int kXcds = 8;
 
 template <int STORE>
 __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
-    __shared__ UpdParams s_p[kMaxBatchScenes];
     __shared__ int s_prefix[kMaxBatchScenes + 1];
     const int waves = blockDim.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int k = b.step_ctr ? *b.step_ctr : b.step;
     const int ns = b.n_scenes;
-    if ((int)threadIdx.x < ns) {
-        const int sc = threadIdx.x;
-        const SceneStep& st = b.steps[k * ns + sc];
-        s_p[sc] = make_params(b.vols[sc], st.cam, batch_call(b, sc, k).parity);
-        if (!st.active) s_p[sc].n = 0;
-    }
-    __syncthreads();
+    const UpdParams OSN_CONST* P = (const UpdParams OSN_CONST*)b.upd;
     if (threadIdx.x == 0) {
         int acc = 0;
-        for (int sc = 0; sc < ns; ++sc) { s_prefix[sc] = acc; acc += s_p[sc].n; }
+        for (int sc = 0; sc < ns; ++sc) { s_prefix[sc] = acc; acc += P[sc].n; }
         s_prefix[ns] = acc;
     }
     __syncthreads();
@@ -347,19 +299,18 @@ __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
     int item = lo + (int)(blockIdx.x / kXcds) * waves + wave_id();
     int sc = 0, lanes_sc = -1;
     while (item < hi && s_prefix[sc + 1] <= item) ++sc;
-    uint32_t e = item < hi ? load_touch(s_p[sc], item - s_prefix[sc], lane) : 0u;
+    uint32_t e = item < hi ? load_touch(P[sc], item - s_prefix[sc], lane) : 0u;
     LaneRuns L{};
     while (item < hi) {  // the next block's entry is requested a block ahead
         const int next = item + stride;
         int scn = sc;  // items ascend, so the scene only moves forward
         while (next < hi && s_prefix[scn + 1] <= next) ++scn;
-        const uint32_t en = next < hi ? load_touch(s_p[scn], next - s_prefix[scn], lane) : 0u;
-        const UpdParams& p = s_p[sc];
+        const uint32_t en = next < hi ? load_touch(P[scn], next - s_prefix[scn], lane) : 0u;
         if (sc != lanes_sc) {  // the lane offsets depend on the scene's pose
-            L = lane_runs(p, lane);
+            L = lane_runs(P[sc], lane);
             lanes_sc = sc;
         }
-        update_block<STORE>(p, L, e, lane);
+        update_block<STORE>(P[sc], L, e, lane);
         e = en;
         item = next;
         sc = scn;
