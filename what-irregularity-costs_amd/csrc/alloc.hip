@@ -58,6 +58,43 @@ extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_or_
 namespace osn_hip {
 namespace {
 
+// Table and list accesses go through the global address space (global, not
+// flat, instructions: a flat access also counts against the LDS wait counter,
+// so every LDS wait of the pre-hash would wait for outstanding table probes).
+#define OSN_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ inline T OSN_GLOBAL* glob(T* p) {
+    return (T OSN_GLOBAL*)p;
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+#define OSN_CONST __attribute__((address_space(4)))
+
+// Copy of a descriptor that stays constant during the kernel, read through the
+// constant address space (scalar loads, uniform over the wave).
+template <typename T>
+__device__ inline T load_const(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "descriptor of whole dwords");
+    T out;
+    const uint32_t OSN_CONST* src = (const uint32_t OSN_CONST*)p;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+    return out;
+}
+
+// Store a 16-byte list entry with one global store.
+__device__ inline void put(TouchEntry* p, const TouchEntry& e) {
+    u32x4 w;
+    __builtin_memcpy(&w, &e, sizeof(w));
+    *(u32x4 OSN_GLOBAL*)p = w;
+}
+
+__device__ inline int32_t add_global(int32_t* p, int32_t x) {
+    return __hip_atomic_fetch_add(glob(p), x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Wave-wide sum (every lane active).
 __device__ inline unsigned long long wave_sum_u64(unsigned long long x) {
     return __ockl_wfred_add_u64(x);
@@ -75,9 +112,9 @@ struct WgShared {
 
 // Append `mine` (when `want`) to an LDS list, one LDS atomic per wave; past
 // the LDS capacity, straight to the global list.
-template <int CAP, typename T>
-__device__ inline void wave_append(bool want, T mine, int32_t* lds_count, T* lds_list,
-                                   int32_t* g_count, T* g_list) {
+template <int CAP>
+__device__ inline void wave_append(bool want, TouchEntry mine, int32_t* lds_count,
+                                   TouchEntry* lds_list, int32_t* g_count, TouchEntry* g_list) {
     const unsigned long long m = __ballot(want);
     if (m == 0) return;
     const int leader = __ffsll((long long)m) - 1;
@@ -87,7 +124,7 @@ __device__ inline void wave_append(bool want, T mine, int32_t* lds_count, T* lds
     if (!want) return;
     const int i = base + __popcll(m & lanemask_lt());
     if (i < CAP) lds_list[i] = mine;
-    else g_list[atomicAdd(g_count, 1)] = mine;
+    else put(g_list + add_global(g_count, 1), mine);
 }
 
 // Publish the workgroup's buffered lists: one global atomic per list.
@@ -95,13 +132,13 @@ __device__ void flush_lists(const VolDev& v, ListCounters* lc, WgShared& s) {
     __syncthreads();
     if (threadIdx.x == 0) {
         const int nt = min(s.n_touched, kWgList), nn = min(s.n_new, kWgNew);
-        s.base_touched = nt ? atomicAdd(&lc->touched_count, nt) : 0;
-        s.base_new = nn ? atomicAdd(&lc->new_count, nn) : 0;
+        s.base_touched = nt ? add_global(&lc->touched_count, nt) : 0;
+        s.base_new = nn ? add_global(&lc->new_count, nn) : 0;
     }
     __syncthreads();
     const int nt = min(s.n_touched, kWgList), nn = min(s.n_new, kWgNew);
-    for (int i = threadIdx.x; i < nt; i += blockDim.x) v.touched[s.base_touched + i] = s.touched[i];
-    for (int i = threadIdx.x; i < nn; i += blockDim.x) v.new_keys[s.base_new + i] = s.new_keys[i];
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) put(v.touched + s.base_touched + i, s.touched[i]);
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) put(v.new_keys + s.base_new + i, s.new_keys[i]);
     __syncthreads();
     if (threadIdx.x == 0) { s.n_touched = 0; s.n_new = 0; }
     __syncthreads();
@@ -178,21 +215,22 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
             // key, index and stamp of the slot in one round trip: no index
             // changes during this phase, and a key inserted during it still
             // holds kIdxPending and stamp 0 from the empty slot
+            HashEntry OSN_GLOBAL* ent = glob(v.table) + slot;
             const int64_t cur =
-                __hip_atomic_load(&v.table[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int32_t ci = __hip_atomic_load(&v.table[slot].block_idx, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            const int32_t cs = __hip_atomic_load(&v.table[slot].stamp, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_load(&ent->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int32_t ci =
+                __hip_atomic_load(&ent->block_idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int32_t cs =
+                __hip_atomic_load(&ent->stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (cur == key) { found = true; idx = ci; stamp = cs; break; }
             if (cur == kEmptyKey) {
                 if (MODE == kAllocNoCas) break;
                 if (MODE == kAllocCountCas) cas += 1;
-                const unsigned long long prev =
-                    atomicCAS(reinterpret_cast<unsigned long long*>(&v.table[slot].key),
-                              (unsigned long long)kEmptyKey, (unsigned long long)key);
-                if (prev == (unsigned long long)kEmptyKey) { inserted = true; break; }
-                if (prev == (unsigned long long)key) { found = true; break; }
+                int64_t prev = kEmptyKey;  // the slot's key after the exchange attempt
+                __hip_atomic_compare_exchange_strong(&ent->key, &prev, key, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (prev == kEmptyKey) { inserted = true; break; }
+                if (prev == key) { found = true; break; }
             }
             slot = next_slot(v, slot);
         }
@@ -228,7 +266,8 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
     // did, and needs no exchange.
     bool touch = false;
     if (TOUCH && (found || inserted) && stamp != c.stamp) {
-        const int32_t old = atomicExch(&v.table[slot].stamp, c.stamp);
+        const int32_t old = __hip_atomic_exchange(&glob(v.table)[slot].stamp, c.stamp,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         touch = (old != c.stamp);
     }
     if (TOUCH)
@@ -413,7 +452,7 @@ __device__ inline void walk_points_any(const TileCtx& c, const PointSrc& src, bo
 
 // Depth of pixel (u, w), 0 (invalid) outside the image.
 __device__ inline float load_depth(const PointSrc& src, int u, int w) {
-    return (u < src.W && w < src.H) ? src.depth[(size_t)w * src.W + u] : 0.0f;
+    return (u < src.W && w < src.H) ? glob(src.depth)[(size_t)w * src.W + u] : 0.0f;
 }
 
 // A depth pixel is back-projected in FP32 as
@@ -469,15 +508,16 @@ __device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t*
         // Given keys have no spatial coherence to exploit within a wave, so
         // each lane goes straight to the workgroup pre-hash with its key.
         const int64_t key =
-            has ? pack_coord(coords[i * 3 + 0], coords[i * 3 + 1], coords[i * 3 + 2]) : kEmptyKey;
+            has ? pack_coord(glob(coords)[i * 3 + 0], glob(coords)[i * 3 + 1], glob(coords)[i * 3 + 2])
+                : kEmptyKey;
         process_keys<MODE, TOUCH>(c, has, key);
         return;
     }
     float px = 0.f, py = 0.f, pz = 0.f;
     if (has) {
-        px = src.pos[i * 3 + 0];
-        py = src.pos[i * 3 + 1];
-        pz = src.pos[i * 3 + 2];
+        px = glob(src.pos)[i * 3 + 0];
+        py = glob(src.pos)[i * 3 + 1];
+        pz = glob(src.pos)[i * 3 + 2];
     }
     walk_points_any<MODE, TOUCH>(c, src, has, px, py, pz);
 }
@@ -563,25 +603,27 @@ __global__ __launch_bounds__(kAllocThreads) void k_alloc_batch(BatchDev b, int32
     int sc = 0;
     CallInfo call{0, 0};
     VolDev v{};
+    PointSrc src{};
     for (int tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
         while (sc + 1 < b.n_scenes && s_prefix[sc + 1] <= tile) ++sc;  // tiles ascend
-        const SceneStep& st = b.steps[k * b.n_scenes + sc];
+        const SceneStep* st = b.steps + k * b.n_scenes + sc;
         const int local = tile - s_prefix[sc];
         if (local == 0 && threadIdx.x == 0)
             reset_other_parity(b.vols[sc].ctr, batch_call(b, sc, k).parity);
-        if (!st.active || local >= st.tiles) continue;
+        if (!load_const(&st->active) || local >= load_const(&st->tiles)) continue;
         if (sc != cur) {
             if (cur >= 0) {
                 flush_lists(v, &v.ctr->lists[call.parity], s);
                 flush_stats<kAllocNormal>(v, s);
             }
             cur = sc;
-            v = b.vols[sc];
+            v = load_const(b.vols + sc);
+            src = load_const(&st->src);
             call = batch_call(b, sc, k);
         }
         clear_set(s, set_size);
         const TileCtx c{&v, &v.ctr->lists[call.parity], &s, set_size, call.stamp};
-        alloc_depth_unit<kAllocNormal, true>(c, st.src, local, kBatchDepthSub);
+        alloc_depth_unit<kAllocNormal, true>(c, src, local, kBatchDepthSub);
     }
     if (cur >= 0) {
         flush_lists(v, &v.ctr->lists[call.parity], s);
