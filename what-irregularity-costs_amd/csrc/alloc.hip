@@ -1088,10 +1088,12 @@ hipError_t launch_assign(const VolDev& v, CallInfo call, hipStream_t s) {
 }
 
 hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStream_t s) {
-    // items are taken by ticket, so any width is correct; a scene's first
-    // frame is the only one with more than a few tiles of new keys
+    // Items are taken by ticket, so any width is correct. A frame after a
+    // scene's first few adds a few hundred new keys (one tile), so a narrow
+    // launch: dispatching idle 1024-thread workgroups was measured to cost
+    // more than the first frames gain from a wide one.
     const uint32_t tiles = (uint32_t)assign_tiles_max(max_capacity);
-    const uint32_t gx = tiles < 32u ? tiles : 32u;
+    const uint32_t gx = tiles < 4u ? tiles : 4u;
     k_assign_batch<<<dim3(gx, b.n_scenes), 1024, 0, s>>>(b);
     return hipGetLastError();
 }
