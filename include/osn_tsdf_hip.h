@@ -256,8 +256,23 @@ void osn_tsdf_hip_batch_synchronize(const OsnTsdfHipBatch* b);
  * workgroup size (0 keeps the default). Invalidates a captured graph. */
 int32_t osn_tsdf_hip_batch_tune(OsnTsdfHipBatch* b, int32_t alloc_grid, int32_t update_grid,
                                 int32_t update_threads, int32_t lds_set);
-/* Stage timing for batched steps, as osn_tsdf_hip_read_timing (stages 0-2;
- * with use_graph the whole step is reported as stage 0). */
+/* Mesh extraction inside batched steps: after every `every`-th step (step
+ * numbers count from the schedule's start; 0 turns extraction off) scene s's
+ * triangle soup goes to d_posnor[s] (6 floats per vertex, capacity
+ * vert_caps[s] vertices) and its vertex count, or -1 on overflow, to
+ * d_counts[s], all device memory owned by the caller. The count, scan and
+ * emit passes are captured with the step (a second graph for the steps that
+ * extract) and read block counts on the device, so nothing synchronises.
+ * Extends osn_tsdf_cuda_extract_mesh (c_api.h:96-98) to the batched, device-
+ * resident case; same min_weight/iso meaning. 0 ok, 1 bad arguments, 2 device
+ * error. */
+int32_t osn_tsdf_hip_batch_set_extract(OsnTsdfHipBatch* b, float* const* d_posnor,
+                                       const int32_t* vert_caps, int32_t* d_counts,
+                                       float min_weight, float iso, int32_t every);
+
+/* Stage timing for batched steps, as osn_tsdf_hip_read_timing (stages 0-2,
+ * and 3 for the extraction after a step; with use_graph the whole step is
+ * reported as stage 0). */
 int32_t osn_tsdf_hip_batch_set_timing(OsnTsdfHipBatch* b, int32_t enable);
 int32_t osn_tsdf_hip_batch_read_timing(OsnTsdfHipBatch* b, double* ms4, int64_t* calls4);
 

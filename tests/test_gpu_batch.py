@@ -153,3 +153,47 @@ def test_batch_rejects_bad_groups(H):
     assert a.integrate_depth(d, fr) == 3  # single calls refused while batched
     b.close()
     assert a.integrate_depth(d, fr) == 0
+
+
+def _same_mesh(a, b, count):
+    """Bitwise equality of the first `count` vertices, on the device (noisy
+    depth makes triangle soups of millions of vertices)."""
+    n = int(count) * 6
+    return bool(torch.equal(a[:n].view(torch.int32), b[:n].view(torch.int32)))
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batched_extraction_equals_per_scene(H, S, use_graph):
+    """Meshes extracted inside batched steps (every 3rd step, graph-captured
+    or eager) are byte-identical to each scene's own device extraction of the
+    same state; a buffer too small reports -1."""
+    n_steps = 6
+    depths, frames = make_inputs(H, S, n_steps, seed0=110)
+    solo = volumes(H, 2)
+    batched = volumes(H, 2)
+    b = H.Batch(batched)
+    assert b.set_schedule(depths, frames) == 0
+    caps = [1 << 25, 1 << 25, 64]  # the last scene's mesh does not fit
+    bufs = [torch.empty(c * 6, dtype=torch.float32, device="cuda") for c in caps]
+    counts = torch.full((3,), -7, dtype=torch.int32, device="cuda")
+    assert b.set_extract(bufs, caps, counts, 0.5, 0.0, every=3) == 0
+    for k in range(n_steps):
+        for s, v in enumerate(solo):
+            if depths[k][s] is not None:
+                assert v.integrate_depth(depths[k][s], frames[k][s]) == 0
+        assert b.run(1, use_graph=use_graph) == 0
+        b.synchronize()
+        if k % 3 != 2:
+            continue
+        got = counts.cpu().numpy()
+        assert got[2] == -1
+        for s in range(2):
+            ref = torch.empty(caps[s] * 6, dtype=torch.float32, device="cuda")
+            rc = torch.zeros(1, dtype=torch.int32, device="cuda")
+            assert solo[s].extract_device(ref, None, caps[s], 0.5, 0.0, rc) == 0
+            solo[s].synchronize()
+            assert got[s] == int(rc.item()) > 0
+            assert _same_mesh(bufs[s], ref, got[s])
+            del ref
+        counts.fill_(-7)
+    b.close()

@@ -815,6 +815,12 @@ struct OsnTsdfHipBatch {
     int32_t alloc_grid = 0, update_grid = 0, update_threads = 256, lds_set = 1024;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // mesh extraction every `extract_every` steps (0: never), per scene into
+    // caller-owned device buffers; steps that extract replay their own graph
+    std::vector<ExtractArgs> ex_args;
+    int32_t extract_every = 0;
+    hipGraph_t graph_x = nullptr;
+    hipGraphExec_t exec_x = nullptr;
     bool timing = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double ms[kStages] = {0, 0, 0, 0};
@@ -823,8 +829,57 @@ struct OsnTsdfHipBatch {
     void drop_graph() {
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
+        if (exec_x) hipGraphExecDestroy(exec_x);
+        if (graph_x) hipGraphDestroy(graph_x);
         exec = nullptr;
         graph = nullptr;
+        exec_x = nullptr;
+        graph_x = nullptr;
+    }
+    bool extracts_after(int32_t step) const {
+        return extract_every > 0 && (step + 1) % extract_every == 0;
+    }
+    // Count, scan and emit every scene's mesh (device-side block counts, no
+    // host synchronisation).
+    hipError_t enqueue_extract() {
+        hipEvent_t e[2] = {};
+        if (timing && hipEventCreate(&e[0]) == hipSuccess) hipEventRecord(e[0], stream);
+        hipError_t err = hipSuccess;
+        for (size_t s = 0; s < vols.size() && err == hipSuccess; ++s) {
+            err = launch_extract(vols[s]->d, ex_args[s], false, stream);
+            if (err == hipSuccess) err = launch_extract(vols[s]->d, ex_args[s], true, stream);
+        }
+        if (timing && hipEventCreate(&e[1]) == hipSuccess) {
+            hipEventRecord(e[1], stream);
+            ev_pending.push_back({3, {e[0], e[1]}});
+        }
+        return err;
+    }
+    // Capture one step (and the extraction after it when `with_extract`) as
+    // a graph whose kernels read the step number from d_step.
+    bool capture(bool with_extract, hipGraph_t& g_out, hipGraphExec_t& e_out) {
+        const bool t = timing;
+        timing = false;
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            timing = t;
+            return false;
+        }
+        hipError_t err = enqueue_step(dev(0, true));
+        if (err == hipSuccess && with_extract) err = enqueue_extract();
+        if (err == hipSuccess) err = launch_step_advance(d_step, stream);
+        hipGraph_t g = nullptr;
+        const hipError_t end = hipStreamEndCapture(stream, &g);
+        timing = t;
+        if (err != hipSuccess || end != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return false;
+        }
+        if (hipGraphInstantiate(&e_out, g, nullptr, nullptr, 0) != hipSuccess) {
+            hipGraphDestroy(g);
+            return false;
+        }
+        g_out = g;
+        return true;
     }
     BatchDev dev(int step, bool from_device) const {
         BatchDev b{};
@@ -955,33 +1010,27 @@ int32_t osn_tsdf_hip_batch_run(OsnTsdfHipBatch* b, int32_t n_steps, int32_t use_
     if (!b || n_steps <= 0 || !b->d_steps) return 1;
     if (b->cursor + n_steps > b->n_steps) return 1;
     if (!use_graph) {
-        for (int i = 0; i < n_steps; ++i)
+        for (int i = 0; i < n_steps; ++i) {
             if (b->enqueue_step(b->dev(b->cursor + i, false)) != hipSuccess) return 2;
+            if (b->extracts_after(b->cursor + i) && b->enqueue_extract() != hipSuccess) return 2;
+        }
     } else {
         const int32_t first = b->cursor;
         if (hipMemcpyAsync(b->d_step, &first, sizeof(int32_t), hipMemcpyHostToDevice, b->stream) ||
             hipStreamSynchronize(b->stream))
             return 2;
-        if (!b->exec) {
-            const bool t = b->timing;
-            b->timing = false;
-            if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return 2;
-            hipError_t err = b->enqueue_step(b->dev(0, true));
-            if (err == hipSuccess) err = launch_step_advance(b->d_step, b->stream);
-            hipGraph_t g = nullptr;
-            const hipError_t end = hipStreamEndCapture(b->stream, &g);
-            b->timing = t;
-            if (err != hipSuccess || end != hipSuccess) { if (g) hipGraphDestroy(g); return 2; }
-            b->graph = g;
-            if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
-                b->drop_graph();
-                return 2;
-            }
+        if (!b->exec && !b->capture(false, b->graph, b->exec)) { b->drop_graph(); return 2; }
+        if (b->extract_every > 0 && !b->exec_x &&
+            !b->capture(true, b->graph_x, b->exec_x)) {
+            b->drop_graph();
+            return 2;
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (b->timing) { hipEventCreate(&e0); hipEventRecord(e0, b->stream); }
-        for (int i = 0; i < n_steps; ++i)
-            if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) return 2;
+        for (int i = 0; i < n_steps; ++i) {
+            hipGraphExec_t x = b->extracts_after(first + i) ? b->exec_x : b->exec;
+            if (hipGraphLaunch(x, b->stream) != hipSuccess) return 2;
+        }
         if (b->timing) {
             hipEventCreate(&e1);
             hipEventRecord(e1, b->stream);
@@ -1009,6 +1058,41 @@ int32_t osn_tsdf_hip_batch_tune(OsnTsdfHipBatch* b, int32_t alloc_grid, int32_t 
     if (update_grid >= 0) b->update_grid = update_grid;
     if (update_threads) b->update_threads = update_threads;
     if (lds_set >= 0) b->lds_set = lds_set;
+    return 0;
+}
+
+int32_t osn_tsdf_hip_batch_set_extract(OsnTsdfHipBatch* b, float* const* d_posnor,
+                                       const int32_t* vert_caps, int32_t* d_counts,
+                                       float min_weight, float iso, int32_t every) {
+    if (!b || every < 0) return 1;
+    const size_t n = b->vols.size();
+    if (every > 0 && (!d_posnor || !vert_caps || !d_counts)) return 1;
+    if (every > 0)
+        for (size_t s = 0; s < n; ++s)
+            if (!d_posnor[s] || vert_caps[s] <= 0) return 1;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return 2;
+    b->drop_graph();
+    b->extract_every = every;
+    b->ex_args.assign(n, ExtractArgs{});
+    if (every == 0) return 0;
+    for (size_t s = 0; s < n; ++s) {
+        OsnTsdfHipVolume* v = b->vols[s];
+        // scratch for a full pool, allocated now: nothing allocates while a
+        // graph is captured or replayed
+        if (!v->grow_extract(v->d.pool_capacity)) { v->ok = false; return 2; }
+        ExtractArgs& a = b->ex_args[s];
+        a.min_weight = min_weight;
+        a.iso = iso;
+        a.n_blocks = -1;
+        a.block_vcount = v->ex_vcount;
+        a.block_voff = v->ex_voff;
+        a.chunk_sums = v->ex_chunks;
+        a.scan_flag = v->ex_flag_count;
+        a.posnor = d_posnor[s];
+        a.rgb = nullptr;
+        a.cap = vert_caps[s];
+        a.count_out = d_counts + s;
+    }
     return 0;
 }
 

@@ -124,6 +124,10 @@ def lib():
         "osn_tsdf_hip_batch_tune": (_I, [_VP, _I, _I, _I, _I]),
         "osn_tsdf_hip_batch_set_timing": (_I, [_VP, _I]),
         "osn_tsdf_hip_batch_read_timing": (_I, [_VP, _VP, _VP]),
+        "osn_tsdf_hip_batch_set_extract": (_I, [_VP, _VP, _VP, _VP, _F, _F, _I]),
+        "osn_tsdf_hip_insert_blocks_mode": (_I, [_VP, _VP, _I, _I]),
+        "osn_tsdf_hip_cas_attempts": (_I, [_VP, _VP]),
+        "osn_tsdf_hip_cas_ceiling": (_I, [_I, _I, _VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -379,37 +383,35 @@ class Batch:
         self._L.osn_tsdf_hip_batch_read_timing(self.h, _hptr(ms), _hptr(calls))
         return ms, calls
 
-
-def _bind_extra(L):
-    extra = {
-        "osn_tsdf_hip_insert_blocks_mode": (_I, [_VP, _VP, _I, _I]),
-        "osn_tsdf_hip_cas_attempts": (_I, [_VP, _VP]),
-        "osn_tsdf_hip_cas_ceiling": (_I, [_I, _I, _VP]),
-    }
-    for name, (res, args) in extra.items():
-        fn = getattr(L, name)
-        fn.restype = res
-        fn.argtypes = args
-    return L
+    def set_extract(self, d_posnor, vert_caps, d_counts, min_weight=0.5, iso=0.0,
+                    every=1) -> int:
+        """Mesh every scene after every `every`-th step into d_posnor[s]
+        (device float tensors, 6 per vertex); d_counts is a device int32 tensor
+        with one entry per scene (vertex count, -1 on overflow). every=0 stops."""
+        n = len(self.volumes)
+        if every == 0:
+            return self._L.osn_tsdf_hip_batch_set_extract(self.h, None, None, None, 0.0, 0.0, 0)
+        self._ex_keep = (list(d_posnor), d_counts)  # the buffers must outlive their use
+        ptrs = (C.c_void_p * n)(*[_dptr(t) for t in d_posnor])
+        caps = (C.c_int32 * n)(*[int(c) for c in vert_caps])
+        return self._L.osn_tsdf_hip_batch_set_extract(self.h, ptrs, caps, _dptr(d_counts),
+                                                     float(min_weight), float(iso), int(every))
 
 
 def insert_blocks_mode(vol: HipVolume, d_coords, mode: int) -> int:
-    L = _bind_extra(lib())
-    return L.osn_tsdf_hip_insert_blocks_mode(vol.h, _dptr(d_coords), d_coords.numel() // 3, mode)
+    return lib().osn_tsdf_hip_insert_blocks_mode(vol.h, _dptr(d_coords), d_coords.numel() // 3, mode)
 
 
 def cas_attempts(vol: HipVolume) -> int:
-    L = _bind_extra(lib())
     out = C.c_uint64(0)
-    if L.osn_tsdf_hip_cas_attempts(vol.h, C.addressof(out)):
+    if lib().osn_tsdf_hip_cas_attempts(vol.h, C.addressof(out)):
         raise RuntimeError("cas_attempts failed")
     return out.value
 
 
 def cas_ceiling(log2_slots: int, reps: int = 5) -> float:
     """Milliseconds for 2^log2_slots uncontended, scattered 64-bit CASes."""
-    L = _bind_extra(lib())
     ms = C.c_double(0.0)
-    if L.osn_tsdf_hip_cas_ceiling(log2_slots, reps, C.addressof(ms)):
+    if lib().osn_tsdf_hip_cas_ceiling(log2_slots, reps, C.addressof(ms)):
         raise RuntimeError("cas_ceiling failed")
     return ms.value
