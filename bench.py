@@ -17,6 +17,9 @@ Reported:
                  voxels x (read + write) x bytes/voxel + depth image + block
                  list) / its average duration from HIP events on the volume's
                  stream, against 8 TB/s;
+  chain_*        the same frames with a device-resident mesh extraction of
+                 every scene after every frame (allocate -> assign -> update
+                 -> extract), batched fp16 against plain fp32;
   cpu_baseline   the reference CPU arm (compiled from the reference sources,
                  oracle/_ref) integrating the same frames' back-projected points
                  with all host threads, on a bounded sample of frames.
@@ -157,6 +160,56 @@ def run_batched_instrumented(H, cfg, storage, scenes, warmup, steps, bytes_per_v
     for v in vols:
         v.close()
     return ms, calls, tb, alg_bytes
+
+
+CHAIN_VERTS = 1 << 25  # per-scene mesh buffer (vertices); noisy 1 cm fields make dense soups
+
+
+def run_chain(H, cfg, storage, scenes, warmup, steps, batched):
+    """The full per-frame chain allocate -> assign -> update -> extract (mesh of
+    every scene into a device buffer after every frame): batched in one graph
+    per step, or one scene at a time through the single-volume calls.
+    Returns (seconds, vertex counts of the last meshes; -1 = overflow)."""
+    import torch
+    n = len(scenes)
+    bufs = [torch.empty(CHAIN_VERTS * 6, dtype=torch.float32, device="cuda") for _ in range(n)]
+    counts = torch.zeros(n, dtype=torch.int32, device="cuda")
+    if batched:
+        b, vols = make_batch(H, cfg, storage, scenes)
+        if b.set_extract(bufs, [CHAIN_VERTS] * n, counts, 0.5, 0.0, every=1):
+            raise RuntimeError("set_extract failed")
+        if b.run(warmup, use_graph=True):
+            raise RuntimeError("batch run failed")
+        b.synchronize()
+        t0 = time.perf_counter()
+        if b.run(steps, use_graph=True):
+            raise RuntimeError("batch run failed")
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        b.close()
+    else:
+        dt = 0.0
+        vols = []
+        for s, (frames, depth) in enumerate(scenes):
+            v = make_volume(H, cfg, storage)
+            vols.append(v)
+            for k in range(warmup):
+                if v.integrate_depth(depth[k], frames[k]):
+                    raise RuntimeError("integrate failed")
+            v.synchronize()
+            t0 = time.perf_counter()
+            for k in range(warmup, warmup + steps):
+                if v.integrate_depth(depth[k], frames[k]) or v.extract_device(
+                        bufs[s], None, CHAIN_VERTS, 0.5, 0.0, counts[s:s + 1]):
+                    raise RuntimeError("integrate/extract failed")
+            v.synchronize()
+            dt += time.perf_counter() - t0
+    out = counts.cpu().tolist()
+    for v in vols:
+        v.close()
+    del bufs
+    torch.cuda.empty_cache()
+    return dt, out
 
 
 def precision(H, cfg, frames, depth, n):
@@ -307,6 +360,22 @@ def main():
             ("plain_fp16", dt_p16))}
     log(" ".join(f"{k} {v:.0f}" for k, v in fps.items()))
 
+    log("chain with extraction every frame (timed)")
+    chain_steps = min(args.steps, 50)
+    dt_cb, verts_b = run_chain(H, cfg, H.STORE_F16, scenes, args.warmup, chain_steps, True)
+    dt_cp, verts_p = run_chain(H, cfg, H.STORE_F32, scenes, args.warmup, chain_steps, False)
+    chain = {
+        "stages": "allocate + assign + update + extract (device mesh) per frame",
+        "steps": chain_steps,
+        "batched_fp16_frames_per_s": round(chain_steps * args.scenes / dt_cb, 1),
+        "plain_fp32_frames_per_s": round(chain_steps * args.scenes / dt_cp, 1),
+        "last_mesh_vertices_fp16": verts_b,
+        "last_mesh_vertices_fp32": verts_p,
+    }
+    chain["ratio"] = round(chain["batched_fp16_frames_per_s"] / chain["plain_fp32_frames_per_s"], 3)
+    log(f"chain batched fp16 {chain['batched_fp16_frames_per_s']:.0f} plain fp32 "
+        f"{chain['plain_fp32_frames_per_s']:.0f}")
+
     log("instrumented batched runs")
     ms16, calls16, tb16, bytes16 = run_batched_instrumented(H, cfg, H.STORE_F16, scenes,
                                                             args.warmup, args.steps, 4)
@@ -355,6 +424,7 @@ def main():
         "touched_blocks_per_step": round(float(tb16.mean()), 1),
         "update_fp32_gbs": round(float(achieved32), 1),
         "precision_vs_fp64": prec,
+        "chain_with_extraction": chain,
         "roofline": {"bound": "hbm", "achieved": round(float(achieved16), 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(float(achieved16 / HBM_PEAK_GBS), 4), "traffic": None,
