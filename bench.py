@@ -212,8 +212,15 @@ def run_chain(H, cfg, storage, scenes, warmup, steps, batched):
     return dt, out
 
 
+PRECISION_MESH_VERTS = 40 << 20
+
+
 def precision(H, cfg, frames, depth, n):
-    """FP16 and FP32 states against the FP64 oracle kernel after n frames."""
+    """FP16 and FP32 states against the FP64 oracle kernel after n frames:
+    |dtsdf| over every observed voxel, and the extracted meshes (min weight
+    0.5, iso 0) against the FP64 mesh through the mesh-comparison gate
+    (Chamfer mean = symmetric mean surface distance, Hausdorff, p99)."""
+    import mesh_metrics as MM
     vols = {}
     for name, st in (("f64", H.STORE_F64), ("f32", H.STORE_F32), ("f16", H.STORE_F16)):
         v = make_volume(H, cfg, st)
@@ -221,6 +228,7 @@ def precision(H, cfg, frames, depth, n):
             v.integrate_depth(depth[k], frames[k])
         v.synchronize()
         vols[name] = v
+    meshes = {name: v.extract(0.5, 0.0, cap=PRECISION_MESH_VERTS) for name, v in vols.items()}
     keys = vols["f64"].keys()
     t64, w64 = vols["f64"].read_voxels_f64()
     vols["f64"].close()
@@ -237,6 +245,14 @@ def precision(H, cfg, frames, depth, n):
                      "max_voxels": float(err.max() * vox),
                      "rms_voxels": float(np.sqrt((err ** 2).mean()) * vox),
                      "observed_voxels": int(obs.sum())}
+        if meshes[name] is not None and meshes["f64"] is not None:
+            ag = MM.compare(meshes[name], meshes["f64"], 200000, cfg["voxel"], 0xC0FFEE)
+            out[name]["mesh_vs_fp64"] = {
+                "chamfer_mean_voxels": ag.mean_surface / cfg["voxel"],
+                "hausdorff_p99_voxels": ag.hausdorff_p99 / cfg["voxel"],
+                "hausdorff_voxels": ag.hausdorff / cfg["voxel"],
+                "vertices": int(len(meshes[name])), "vertices_fp64": int(len(meshes["f64"])),
+                "samples_per_side": 200000}
     return out
 
 
