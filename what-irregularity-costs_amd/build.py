@@ -58,7 +58,27 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_harness(verbose)
     return LIB
+
+
+HARNESS_SRC = os.path.join(HERE, "harness", "osn_harness.cpp")
+HARNESS = os.path.join(HERE, "osn_harness")
+
+
+def build_harness(verbose: bool = False) -> str:
+    """The timing harness: host C++ over the C ABI, linked to the library next
+    to it (rpath $ORIGIN, so the pair travels together)."""
+    tmp = HARNESS + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", HARNESS_SRC, "-o", tmp, f"-L{HERE}",
+           "-losn_tsdf_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness build failed:\n{r.stderr}")
+    os.replace(tmp, HARNESS)
+    if verbose:
+        print(f"built {HARNESS}")
+    return HARNESS
 
 
 if __name__ == "__main__":
