@@ -360,6 +360,22 @@ def main():
         dt, _ = run_batched(H, cfg, H.STORE_F16, scenes, args.warmup, args.steps,
                             graph=not args.no_graph)
         log(f"batched fp16 {frames_timed / dt:.0f} frames/s")
+        # Same frames again, one step at a time, for the touched-block counts of
+        # the launches above (identical work per launch, so per-launch counter
+        # averages over both passes are those of the first).
+        b, vols = make_batch(H, cfg, H.STORE_F16, scenes)
+        touched = []
+        for _ in range(args.warmup + args.steps):
+            b.run(1)
+            touched.append(sum(v.touched_count for v in vols))
+        b.close()
+        for v in vols:
+            v.close()
+        tb = np.array(touched, np.float64)
+        W, Hh = cfg["width"], cfg["height"]
+        alg = tb * 512 * 2 * 4 + len(scenes) * W * Hh * 4 + tb * 12
+        log(f"touched blocks per step: mean {tb.mean():.0f} over {len(tb)} steps; "
+            f"update algorithmic bytes per launch {alg.mean() / 1e6:.1f} MB")
         return
 
     log("batched fp16 (timed)")

@@ -964,43 +964,95 @@ __global__ __launch_bounds__(1024) void k_assign_batch(BatchDev b) {
 
 // ---- reference-semantics scatter update (volume.cpp:200-299) ---------------
 
+// The workgroup's visits are summed in LDS per voxel first, then flushed with
+// one set of global atomics per distinct voxel: neighbouring points of a
+// workgroup hit the same voxels many times over, and the global float atomics
+// are what the reference's update spends its time on. Weights are exact
+// integer counts either way; a tsdf sum changes only in summation order,
+// which is arbitrary in the reference too (its atomics race). The colour
+// sums are 128 * count, as the reference's repeated 128s add exactly.
+constexpr int kScatterSet = 2048;  // LDS voxel slots per workgroup (>= 256 x 9 visits / 1.1)
+
+struct ScatterLds {
+    unsigned long long key[kScatterSet];  // voxel index in the pool, ~0 = empty
+    float w[kScatterSet];
+    float t[kScatterSet];
+};
+
+__device__ inline void scatter_global(const VolDev& v, size_t idx, float w, float t) {
+    unsafeAtomicAdd(&v.weight[idx], w);
+    unsafeAtomicAdd(&v.tsdf[idx], t);
+    const float c = 128.0f * w;
+    unsafeAtomicAdd(&v.r[idx], c);
+    unsafeAtomicAdd(&v.g[idx], c);
+    unsafeAtomicAdd(&v.b[idx], c);
+}
+
 __global__ __launch_bounds__(256) void k_scatter_update(VolDev v, PointSrc src) {
+    __shared__ ScatterLds s;
+    for (int e = threadIdx.x; e < kScatterSet; e += blockDim.x) {
+        s.key[e] = ~0ull;
+        s.w[e] = 0.0f;
+        s.t[e] = 0.0f;
+    }
+    __syncthreads();
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= src.n) return;
-    const float px = src.pos[i * 3 + 0], py = src.pos[i * 3 + 1], pz = src.pos[i * 3 + 2];
-    const float dx = px - src.cam[0], dy = py - src.cam[1], dz = pz - src.cam[2];
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    if (src.radius_sq > 0.0f && d2 > src.radius_sq) return;
-    const float dist = sqrtf(d2);
-    if (!(dist > 1e-6f)) return;
-    const float ux = dx / dist, uy = dy / dist, uz = dz / dist;
-    const float inv_trunc = 1.0f / v.trunc;
-    int64_t last_key = kEmptyKey;
-    int32_t last_bi = kIdxAbsent;
-    for (int32_t s = -v.steps; s <= v.steps; ++s) {
-        const float t = (float)s * v.voxel;
-        const int32_t vx = (int32_t)floorf((px + ux * t) * v.inv_voxel);
-        const int32_t vy = (int32_t)floorf((py + uy * t) * v.inv_voxel);
-        const int32_t vz = (int32_t)floorf((pz + uz * t) * v.inv_voxel);
-        const float cx = ((float)vx + 0.5f) * v.voxel;
-        const float cy = ((float)vy + 0.5f) * v.voxel;
-        const float cz = ((float)vz + 0.5f) * v.voxel;
-        const float ex = cx - src.cam[0], ey = cy - src.cam[1], ez = cz - src.cam[2];
-        const float sdf = dist - sqrtf(ex * ex + ey * ey + ez * ez);
-        if (sdf < -v.trunc) continue;
-        const int32_t bx = floor_div8(vx), by = floor_div8(vy), bz = floor_div8(vz);
-        const int64_t key = pack_coord(bx, by, bz);
-        if (key != last_key) { last_key = key; last_bi = find_index(v, key); }
-        if (last_bi < 0) continue;
-        const float sdf_n = fminf(fmaxf(sdf * inv_trunc, -1.0f), 1.0f);
-        const size_t idx = (size_t)last_bi * kBlockVoxels +
-                           (size_t)(((vz & 7) * 8 + (vy & 7)) * 8 + (vx & 7));
-        if (v.weight_cap > 0.0f && v.weight[idx] >= v.weight_cap) continue;
-        unsafeAtomicAdd(&v.weight[idx], 1.0f);
-        unsafeAtomicAdd(&v.tsdf[idx], sdf_n);
-        unsafeAtomicAdd(&v.r[idx], 128.0f);
-        unsafeAtomicAdd(&v.g[idx], 128.0f);
-        unsafeAtomicAdd(&v.b[idx], 128.0f);
+    float px = 0.f, py = 0.f, pz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, d2 = 0.f, dist = 0.f;
+    bool ok = i < src.n;
+    if (ok) {
+        px = glob(src.pos)[i * 3 + 0];
+        py = glob(src.pos)[i * 3 + 1];
+        pz = glob(src.pos)[i * 3 + 2];
+        dx = px - src.cam[0]; dy = py - src.cam[1]; dz = pz - src.cam[2];
+        d2 = dx * dx + dy * dy + dz * dz;
+        ok = !(src.radius_sq > 0.0f && d2 > src.radius_sq);
+        dist = sqrtf(d2);
+        ok = ok && (dist > 1e-6f);
+    }
+    if (ok) {
+        const float ux = dx / dist, uy = dy / dist, uz = dz / dist;
+        const float inv_trunc = 1.0f / v.trunc;
+        int64_t last_key = kEmptyKey;
+        int32_t last_bi = kIdxAbsent;
+        for (int32_t st = -v.steps; st <= v.steps; ++st) {
+            const float t = (float)st * v.voxel;
+            const int32_t vx = (int32_t)floorf((px + ux * t) * v.inv_voxel);
+            const int32_t vy = (int32_t)floorf((py + uy * t) * v.inv_voxel);
+            const int32_t vz = (int32_t)floorf((pz + uz * t) * v.inv_voxel);
+            const float cx = ((float)vx + 0.5f) * v.voxel;
+            const float cy = ((float)vy + 0.5f) * v.voxel;
+            const float cz = ((float)vz + 0.5f) * v.voxel;
+            const float ex = cx - src.cam[0], ey = cy - src.cam[1], ez = cz - src.cam[2];
+            const float sdf = dist - sqrtf(ex * ex + ey * ey + ez * ez);
+            if (sdf < -v.trunc) continue;
+            const int32_t bx = floor_div8(vx), by = floor_div8(vy), bz = floor_div8(vz);
+            const int64_t key = pack_coord(bx, by, bz);
+            if (key != last_key) { last_key = key; last_bi = find_index(v, key); }
+            if (last_bi < 0) continue;
+            const float sdf_n = fminf(fmaxf(sdf * inv_trunc, -1.0f), 1.0f);
+            const size_t idx = (size_t)last_bi * kBlockVoxels +
+                               (size_t)(((vz & 7) * 8 + (vy & 7)) * 8 + (vx & 7));
+            // the reference's racy cap check, against the global weight
+            if (v.weight_cap > 0.0f && glob(v.weight)[idx] >= v.weight_cap) continue;
+            uint32_t h = uint32_t((uint64_t(idx) * 0x9E3779B97F4A7C15ull) >> 40) & (kScatterSet - 1);
+            bool placed = false;
+            for (int p = 0; p < 16; ++p) {
+                const unsigned long long prev = atomicCAS(&s.key[h], ~0ull, (unsigned long long)idx);
+                if (prev == ~0ull || prev == (unsigned long long)idx) {
+                    atomicAdd(&s.w[h], 1.0f);
+                    atomicAdd(&s.t[h], sdf_n);
+                    placed = true;
+                    break;
+                }
+                h = (h + 1) & (kScatterSet - 1);
+            }
+            if (!placed) scatter_global(v, idx, 1.0f, sdf_n);  // LDS table crowded
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kScatterSet; e += blockDim.x) {
+        const unsigned long long k = s.key[e];
+        if (k != ~0ull) scatter_global(v, (size_t)k, s.w[e], s.t[e]);
     }
 }
 
