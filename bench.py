@@ -15,8 +15,9 @@ Reported:
                  frames, in truncation units and in voxels;
   roofline       the update kernel: algorithmic bytes (touched blocks x 512
                  voxels x (read + write) x bytes/voxel + depth image + block
-                 list) / its average duration from HIP events on the volume's
-                 stream, against 8 TB/s;
+                 list) / its average duration from HIP events that its own
+                 launches record (start and end of the dispatch, on the batch
+                 stream), against 8 TB/s;
   chain_*        the same frames with a device-resident mesh extraction of
                  every scene after every frame (allocate -> assign -> update
                  -> extract), batched fp16 against plain fp32;
@@ -133,10 +134,12 @@ def run_batched(H, cfg, storage, scenes, warmup, steps, graph=True):
     return dt, blocks
 
 
-def run_batched_instrumented(H, cfg, storage, scenes, warmup, steps, bytes_per_voxel):
+def run_batched_instrumented(H, cfg, storage, scenes, warmup, steps, bytes_per_voxel,
+                             with_all=False):
     """Per-stage HIP-event times of back-to-back eager batched steps, and the
     per-step touched-block counts (the same frames give the same touched sets,
-    so they are collected in a second, synchronised pass that is not timed)."""
+    so they are collected in a second, synchronised pass that is not timed);
+    with_all: also the counts of every step, warmup included."""
     b, vols = make_batch(H, cfg, storage, scenes)
     b.run(warmup)
     b.synchronize()
@@ -148,17 +151,19 @@ def run_batched_instrumented(H, cfg, storage, scenes, warmup, steps, bytes_per_v
     for v in vols:
         v.close()
     b, vols = make_batch(H, cfg, storage, scenes)
-    b.run(warmup)
     touched = []
-    for _ in range(steps):
+    for _ in range(warmup + steps):
         b.run(1)
         touched.append(sum(v.touched_count for v in vols))  # synchronises
     W, Hh = cfg["width"], cfg["height"]
-    tb = np.array(touched, np.float64)
+    tb_all = np.array(touched, np.float64)
+    tb = tb_all[warmup:]
     alg_bytes = tb * 512 * 2 * bytes_per_voxel + len(scenes) * W * Hh * 4 + tb * 12
     b.close()
     for v in vols:
         v.close()
+    if with_all:
+        return ms, calls, tb, alg_bytes, tb_all
     return ms, calls, tb, alg_bytes
 
 
@@ -360,22 +365,20 @@ def main():
         dt, _ = run_batched(H, cfg, H.STORE_F16, scenes, args.warmup, args.steps,
                             graph=not args.no_graph)
         log(f"batched fp16 {frames_timed / dt:.0f} frames/s")
-        # Same frames again, one step at a time, for the touched-block counts of
-        # the launches above (identical work per launch, so per-launch counter
-        # averages over both passes are those of the first).
-        b, vols = make_batch(H, cfg, H.STORE_F16, scenes)
-        touched = []
-        for _ in range(args.warmup + args.steps):
-            b.run(1)
-            touched.append(sum(v.touched_count for v in vols))
-        b.close()
-        for v in vols:
-            v.close()
-        tb = np.array(touched, np.float64)
+        # The same frames through the instrumented eager pass: per-stage
+        # kernel times from the launches' own events, and the touched-block
+        # counts behind the update's algorithmic bytes.
+        ms, calls, tb, alg, tb_all = run_batched_instrumented(
+            H, cfg, H.STORE_F16, scenes, args.warmup, args.steps, 4, with_all=True)
+        per = ms / np.maximum(calls, 1)
         W, Hh = cfg["width"], cfg["height"]
-        alg = tb * 512 * 2 * 4 + len(scenes) * W * Hh * 4 + tb * 12
-        log(f"touched blocks per step: mean {tb.mean():.0f} over {len(tb)} steps; "
-            f"update algorithmic bytes per launch {alg.mean() / 1e6:.1f} MB")
+        alg_all = tb_all * 512 * 2 * 4 + len(scenes) * W * Hh * 4 + tb_all * 12
+        log(f"stage ms per launch: allocate {per[0]:.5f} assign {per[1]:.5f} update {per[2]:.5f}; "
+            f"touched blocks per step {tb.mean():.0f}; update algorithmic bytes "
+            f"{alg.mean() / 1e6:.1f} MB -> {alg.mean() / (per[2] * 1e-3) / 1e9:.0f} GB/s")
+        log(f"every update launch of this run (3 passes over steps 0..{len(tb_all) - 1}): "
+            f"touched blocks per step {tb_all.mean():.0f}, algorithmic bytes "
+            f"{alg_all.mean() / 1e6:.1f} MB")
         return
 
     log("batched fp16 (timed)")

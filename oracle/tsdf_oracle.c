@@ -677,7 +677,8 @@ int32_t orc_backproject_f32(const float* depth, int32_t W, int32_t H, float fx, 
  *   per run of 4 voxels along x (lx0 = 0 or 4):
  *               off = (ly * F_y + lz * F_z) + lx0 * F_x      (F = A, B, Z)
  *               F(lx0) = F0 + off,  F(lx + 1) = F(lx) + F_x
- *   per voxel:  skip unless Z > 0; iz = 1 / Z; uf = A * iz; vf = B * iz
+ *   per run:    iz(j) for its four Z values by batch inversion (run_reciprocals)
+ *   per voxel:  skip unless 2^-10 <= Z <= 2^14; uf = A * iz; vf = B * iz
  *               skip unless 0 <= uf < W and 0 <= vf < H; pixel = ((int)uf, (int)vf)
  * with each product rounded separately (no FMA).
  */
@@ -696,6 +697,28 @@ typedef struct {
     float z32;         /* FP32 camera z */
     float sdf32;       /* FP32 sdf (after the gates) */
 } ProjDecision;
+
+/* Observable camera depth of a voxel centre, metres, and the reciprocals of a
+ * run of four depths by batch inversion (clamp to the range, running
+ * products, one correctly rounded division, peel off with partial products),
+ * every operation rounded on its own. Restates proj.hip run_reciprocals. */
+#define PROJ_ZMIN (1.0f / 1024.0f)
+#define PROJ_ZMAX 16384.0f
+
+static void run_reciprocals(const float Z[4], float iz[4]) {
+    float z[4];
+    for (int j = 0; j < 4; ++j) z[j] = fminf(fmaxf(Z[j], PROJ_ZMIN), PROJ_ZMAX);
+    const float p1 = z[0] * z[1];
+    const float p2 = p1 * z[2];
+    const float p3 = p2 * z[3];
+    const float r3 = 1.0f / p3;
+    iz[3] = r3 * p2;
+    const float r2 = r3 * z[3];
+    iz[2] = r2 * p1;
+    const float r1 = r2 * z[2];
+    iz[1] = r1 * z[0];
+    iz[0] = r1 * z[1];
+}
 
 static ProjDecision proj_decide(const OrcProjCam* k, const float* depth, float voxel, float trunc,
                                 int32_t bx, int32_t by, int32_t bz, int lx, int ly, int lz) {
@@ -718,18 +741,23 @@ static ProjDecision proj_decide(const OrcProjCam* k, const float* depth, float v
     for (int i = 0; i < 3; ++i)
         d0[i] = k->Rcw[i * 3 + 0] * ox + k->Rcw[i * 3 + 1] * oy + k->Rcw[i * 3 + 2] * oz;
     const float f0[3] = {k->fx * d0[0] + cxh * d0[2], k->fy * d0[1] + cyh * d0[2], d0[2]};
+    /* The voxel's run: the four voxels lx0..lx0+3 along x, whose forms are
+     * the run's first value plus one step per voxel. */
     const int lx0 = lx & 4;
-    float f[3];
+    float run[3][4];
     for (int q = 0; q < 3; ++q) {
         const float off = ((float)ly * g[q][1] + (float)lz * g[q][2]) + (float)lx0 * g[q][0];
-        f[q] = f0[q] + off;
-        for (int j = lx0; j < lx; ++j) f[q] = f[q] + g[q][0];
+        run[q][0] = f0[q] + off;
+        for (int j = 1; j < 4; ++j) run[q][j] = run[q][j - 1] + g[q][0];
     }
-    const float Z = f[2];
-    if (!(Z > 0.0f)) return r;
-    const float iz = 1.0f / Z;
-    const float uf = f[0] * iz;
-    const float vf = f[1] * iz;
+    const int j = lx - lx0;
+    const float Z = run[2][j];
+    if (!(Z >= PROJ_ZMIN) || !(Z <= PROJ_ZMAX)) return r;
+    float izr[4];
+    run_reciprocals(run[2], izr);
+    const float iz = izr[j];
+    const float uf = run[0][j] * iz;
+    const float vf = run[1][j] * iz;
     if (!(uf >= 0.0f) || !(vf >= 0.0f) || !(uf < (float)k->W) || !(vf < (float)k->H)) return r;
     const int32_t u = (int32_t)uf, v = (int32_t)vf;
     const float d = depth[(size_t)v * k->W + u];

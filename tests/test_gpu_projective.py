@@ -116,20 +116,46 @@ def _by_key(v):
     return keys, t, w
 
 
-def test_fp64_kernel_equals_cpu_fp64_oracle(H, S):
+def _other_depth(d, k):
+    """A different image of the same size: depth scaled by 1.004, every 7th
+    pixel (offset k) invalid."""
+    e = d * 1.004
+    e[k % 7::7] = 0.0
+    return e
+
+
+# 157x117: neither side a multiple of the depth tiles (8x4), so the tiled
+# copy's padding and its last partial tiles are exercised. separate=True
+# allocates on one image and updates from another through update_depth, which
+# must tile its own image rather than read the copy allocation left.
+@pytest.mark.parametrize("W,Hh,separate", [(160, 120, False), (157, 117, False),
+                                           (157, 117, True)])
+def test_fp64_kernel_equals_cpu_fp64_oracle(H, S, W, Hh, separate):
     cfg = dict(S.PRECISE, table=1 << 18)
-    _, fr = frames_for(H, S, cfg, 9, 4, W=160, Hh=120)
-    gv = _run(H, H.STORE_F64, cfg, fr, 32.0)
+    _, fr = frames_for(H, S, cfg, 9, 4, W=W, Hh=Hh)
+    if separate:
+        gv = H.HipVolume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 32.0, storage=H.STORE_F64)
+        # the volume's stream reads the images after the calls return: keep
+        # them alive (and written) until it has finished
+        others = [_other_depth(d, k) for k, (f, d, R, t) in enumerate(fr)]
+        torch.cuda.synchronize()
+        for (f, d, R, t), e in zip(fr, others):
+            assert gv.allocate_depth(d, f) == 0
+            assert gv.update_depth(e, f) == 0
+        gv.synchronize()
+    else:
+        gv = _run(H, H.STORE_F64, cfg, fr, 32.0)
     ov = B.Volume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 0.0)
     po = B.ProjOracle(cfg["voxel"], cfg["trunc"], 32, cfg["table"] // 2)
-    for f, d, R, t in fr:
+    for k, (f, d, R, t) in enumerate(fr):
         pts = host_points(H, f, d)
         cam = tuple(f.t_wc[:])
         ov.allocate(pts, cam)
         touched = ov.touched_keys(pts, cam)
         Rwc64 = np.array(f.R_wc64[:]).reshape(3, 3)
         Rwc32 = np.array(f.R_wc[:], np.float32).reshape(3, 3)
-        po.integrate(touched, d.cpu().numpy().reshape(f.height, f.width), f.fx, f.fy, f.cx, f.cy,
+        du = _other_depth(d, k) if separate else d
+        po.integrate(touched, du.cpu().numpy().reshape(f.height, f.width), f.fx, f.fy, f.cx, f.cy,
                      Rwc64.T, np.array(f.t_wc64[:]), Rwc32.T, np.array(f.t_wc[:], np.float32))
     keys = gv.keys()
     t64, w64 = gv.read_voxels_f64()

@@ -141,26 +141,37 @@ def test_backprojection_convention_and_fp64_agreement():
     assert np.allclose(pts.reshape(-1, 3)[k], R[:, 2] * depth[24, 32] + t, atol=1e-5)
 
 
-def test_projective_decision_matches_fp64_projection():
+def _rot(ax_deg, ay_deg):
+    a, b = np.radians(ax_deg), np.radians(ay_deg)
+    rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+    return (rx @ ry).astype(np.float32)
+
+
+# With the identity pose the depth is constant along a run of 4 voxels in x;
+# the rotated poses make it vary, so the run's batch-inverted reciprocals are
+# told apart (a swapped or misindexed reciprocal moves the pixel).
+@pytest.mark.parametrize("ax,ay", [(0.0, 0.0), (10.0, 35.0), (-25.0, -50.0)])
+def test_projective_decision_matches_fp64_projection(ax, ay):
     L = B.lib()
     rng = np.random.default_rng(4)
     W, H = 80, 60
     fx, fy, cx, cy = 40.0, 40.0, 40.0, 30.0
     depth = rng.uniform(0.8, 3.0, size=(H, W)).astype(np.float32)
-    Rcw = np.eye(3, dtype=np.float32)
+    Rcw = _rot(ax, ay)
     c = np.array([0.0, 0.0, -1.0], np.float32)
     voxel, trunc = 0.02, 0.08
     x = np.zeros(1, np.float32)
     agree = 0
     total = 0
-    for _ in range(3000):
+    for _ in range(6000):
         b = rng.integers(-6, 6, 3)
         l = rng.integers(0, 8, 3)
         ok = L.orc_proj_observe_f32(Rcw.ctypes.data_as(B.f32p), c.ctypes.data_as(B.f32p), fx, fy,
                                     cx, cy, W, H, depth.ctypes.data_as(B.f32p), voxel, trunc,
                                     int(b[0]), int(b[1]), int(b[2]), int(l[0]), int(l[1]),
                                     int(l[2]), x.ctypes.data_as(B.f32p))
-        p = (b * 8 + l + 0.5) * voxel - c.astype(np.float64)
+        p = Rcw.astype(np.float64) @ ((b * 8 + l + 0.5) * voxel - c.astype(np.float64))
         if p[2] <= 1e-3:
             assert not ok
             continue

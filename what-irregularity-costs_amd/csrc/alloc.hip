@@ -477,7 +477,11 @@ __device__ inline bool backproject_pixel(const PointSrc& src, int u, int w, floa
 // (sub = 1 for a single frame, whose 16x16 tiles already fill the chip; 2 in
 // batches): the workgroup keeps its pre-hash across the unit's tiles, each of
 // its four waves takes an 8x8 quarter of a tile, and the depth of the next
-// tile is loaded before the current one is walked.
+// tile is loaded before the current one is walked. With src.tiled set, each
+// pixel is also stored to the update's 8x4-tiled copy: a wave's 8x8 quarter is
+// two whole tiles, 256 contiguous bytes. The units cover the image rounded up
+// to 16-pixel multiples, so every pixel of the copy (padding included, as 0)
+// is written.
 template <int MODE, bool TOUCH>
 __device__ void alloc_depth_unit(const TileCtx& c, const PointSrc& src, int unit, int sub) {
     const int span = 16 * sub;
@@ -486,12 +490,15 @@ __device__ void alloc_depth_unit(const TileCtx& c, const PointSrc& src, int unit
     const int u0 = (unit % per_row) * span + (wave & 1) * 8 + (lane & 7);
     const int w0 = (unit / per_row) * span + (wave >> 1) * 8 + (lane >> 3);
     const int n = sub * sub;
+    const int Wt = tiled_width(src.W), Ht = tiled_height(src.H);
+    const uint32_t tk = (uint32_t)tiled_stride_k(src.W);
     float z = load_depth(src, u0, w0);
 #pragma unroll 1
     for (int j = 0; j < n; ++j) {
         const int u = u0 + (j % sub) * 16, w = w0 + (j / sub) * 16;
         const int jn = j + 1;
         const float zn = jn < n ? load_depth(src, u0 + (jn % sub) * 16, w0 + (jn / sub) * 16) : 0.0f;
+        if (src.tiled && u < Wt && w < Ht) glob(src.tiled)[tiled_index(u, w, tk)] = z;
         float px = 0.f, py = 0.f, pz = 0.f;
         const bool ok = backproject_pixel(src, u, w, z, px, py, pz);
         walk_points_any<MODE, TOUCH>(c, src, ok, px, py, pz);
@@ -1124,11 +1131,11 @@ hipError_t launch_alloc(const VolDev& v, const PointSrc& src, const AllocOpts& o
 }
 
 hipError_t launch_alloc_batch(const BatchDev& b, int32_t total_tiles, const AllocOpts& o,
-                              hipStream_t s) {
+                              hipStream_t s, const LaunchEvents& ev) {
     const int set = clamp_set(o.lds_set);
     if (set & (set - 1)) return hipErrorInvalidValue;
     const int grid = o.grid > 0 ? min(total_tiles, o.grid) : total_tiles;
-    k_alloc_batch<<<grid, kAllocThreads, 0, s>>>(b, set, total_tiles);
+    launch_kernel(k_alloc_batch, dim3(grid), dim3(kAllocThreads), s, ev, b, set, total_tiles);
     return hipGetLastError();
 }
 
@@ -1139,14 +1146,15 @@ hipError_t launch_assign(const VolDev& v, CallInfo call, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStream_t s) {
+hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStream_t s,
+                               const LaunchEvents& ev) {
     // Items are taken by ticket, so any width is correct. A frame after a
     // scene's first few adds a few hundred new keys (one tile), so a narrow
     // launch: dispatching idle 1024-thread workgroups was measured to cost
     // more than the first frames gain from a wide one.
     const uint32_t tiles = (uint32_t)assign_tiles_max(max_capacity);
     const uint32_t gx = tiles < 4u ? tiles : 4u;
-    k_assign_batch<<<dim3(gx, b.n_scenes), 1024, 0, s>>>(b);
+    launch_kernel(k_assign_batch, dim3(gx, b.n_scenes), dim3(1024), s, ev, b);
     return hipGetLastError();
 }
 
@@ -1157,6 +1165,30 @@ hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, hipStream
 
 hipError_t launch_fill_table(HashEntry* table, uint32_t n, hipStream_t s) {
     k_fill_table<<<(n + 255) / 256, 256, 0, s>>>(table, n);
+    return hipGetLastError();
+}
+
+namespace {
+
+// Copy a row-major depth image into the update's 8x4-tiled layout, one thread
+// per element of the copy (coalesced stores, 32-byte row pieces loaded).
+__global__ __launch_bounds__(256) void k_tile_depth(const float* depth, int32_t W, int32_t H,
+                                                    float* tiled) {
+    const uint32_t wt8 = (uint32_t)tiled_width(W) >> 3;
+    const uint32_t total = wt8 * 8u * (uint32_t)tiled_height(H);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t t = i >> 5, within = i & 31u;
+    const uint32_t u = (t % wt8) * 8u + (within & 7u), w = (t / wt8) * 4u + (within >> 3);
+    glob(tiled)[i] = (u < (uint32_t)W && w < (uint32_t)H) ? glob(depth)[(size_t)w * W + u] : 0.0f;
+}
+
+}  // namespace
+
+hipError_t launch_tile_depth(const float* depth, int32_t W, int32_t H, float* tiled, hipStream_t s) {
+    if (W <= 0 || H <= 0 || W > kMaxFrameDim || H > kMaxFrameDim) return hipErrorInvalidValue;
+    const uint32_t total = (uint32_t)tiled_width(W) * (uint32_t)tiled_height(H);
+    k_tile_depth<<<(total + 255) / 256, 256, 0, s>>>(depth, W, H, tiled);
     return hipGetLastError();
 }
 

@@ -80,6 +80,9 @@ struct OsnTsdfHipVolume {
     float* stage_posnor = nullptr;
     uint8_t* stage_rgb = nullptr;
     int64_t stage_cap = 0;
+    // the depth image in the update's 8x4-tiled layout, grown on demand
+    float* d_tiled = nullptr;
+    size_t tiled_cap = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -96,6 +99,7 @@ struct OsnTsdfHipVolume {
     void stage_end(int stage, hipEvent_t a);
     bool grow_extract(int32_t blocks);
     bool grow_stage(int64_t verts, bool rgb);
+    bool grow_tiled(int32_t W, int32_t H);
 };
 
 bool OsnTsdfHipVolume::fail(const char* what, hipError_t e) {
@@ -170,7 +174,7 @@ OsnTsdfHipVolume::~OsnTsdfHipVolume() {
     for (auto e : ev_pool) hipEventDestroy(e);
     void* ptrs[] = {d.table, d.ctr, d.block_coord, d.tsdf, d.weight, d.r, d.g, d.b, d.vox,
                     d.new_keys, d.touched, d.sort_a, d.sort_b, d.assign_done, ex_vcount, ex_voff,
-                    ex_chunks, ex_flag_count, stage_posnor, stage_rgb};
+                    ex_chunks, ex_flag_count, stage_posnor, stage_rgb, d_tiled};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (stream) hipStreamDestroy(stream);
@@ -229,6 +233,17 @@ bool OsnTsdfHipVolume::grow_stage(int64_t verts, bool rgb) {
     HIP_OK(hipMalloc(&stage_posnor, (size_t)want * 6 * sizeof(float)));
     if (rgb) HIP_OK(hipMalloc(&stage_rgb, (size_t)want * 3));
     stage_cap = want;
+    return true;
+}
+
+bool OsnTsdfHipVolume::grow_tiled(int32_t W, int32_t H) {
+    const size_t need = (size_t)tiled_width(W) * (size_t)tiled_height(H);
+    if (need <= tiled_cap) return true;
+    if (d_tiled) hipFree(d_tiled);
+    d_tiled = nullptr;
+    tiled_cap = 0;
+    HIP_OK(hipMalloc(&d_tiled, need * sizeof(float)));
+    tiled_cap = need;
     return true;
 }
 
@@ -337,7 +352,8 @@ int32_t run_allocate(OsnTsdfHipVolume* v, const PointSrc& src, int mode, bool to
 }
 
 bool valid_frame(const float* d_depth, const OsnTsdfHipFrame* f) {
-    return d_depth && f && f->width > 0 && f->height > 0 && f->fx != 0.0f && f->fy != 0.0f;
+    return d_depth && f && f->width > 0 && f->height > 0 && f->width <= kMaxFrameDim &&
+           f->height <= kMaxFrameDim && f->fx != 0.0f && f->fy != 0.0f;
 }
 
 // Synchronise, then read one counter. The head of the counters (block_count,
@@ -473,34 +489,58 @@ int32_t osn_tsdf_hip_update_voxels(OsnTsdfHipVolume* v, const float* d_positions
     return 0;
 }
 
-int32_t osn_tsdf_hip_allocate_depth(OsnTsdfHipVolume* v, const float* d_depth,
-                                    const OsnTsdfHipFrame* frame) {
+namespace {
+
+// The allocate walk also leaves the volume's tiled copy of the depth image,
+// which an update of the same frame reads (integrate_depth); a separate
+// update_depth call tiles its own image first.
+int32_t allocate_depth_tiled(OsnTsdfHipVolume* v, const float* d_depth,
+                             const OsnTsdfHipFrame* frame) {
     if (!v || !valid_frame(d_depth, frame)) return 1;
     if (!v->ok) return 2;
-    return run_allocate(v, depth_src(d_depth, frame), kAllocNormal, true);
+    if (!v->in_batch && !v->grow_tiled(frame->width, frame->height)) return 2;
+    PointSrc src = depth_src(d_depth, frame);
+    src.tiled = v->d_tiled;
+    return run_allocate(v, src, kAllocNormal, true);
 }
 
-int32_t osn_tsdf_hip_update_depth(OsnTsdfHipVolume* v, const float* d_depth,
-                                  const OsnTsdfHipFrame* frame) {
+int32_t update_depth_tiled(OsnTsdfHipVolume* v, const float* d_depth,
+                           const OsnTsdfHipFrame* frame, bool tile_first) {
     if (!v || !valid_frame(d_depth, frame)) return 1;
     if (!v->ok) return 2;
     if (v->d.storage == kStoreSums) return 3;
+    if (v->calls == 0 || v->in_batch) return 3;
+    if (!v->grow_tiled(frame->width, frame->height)) return 2;
     hipEvent_t e;
     v->stage_begin(2, e);
-    if (v->calls == 0 || v->in_batch) { v->stage_end(2, e); return 3; }
+    hipError_t err = hipSuccess;
+    if (tile_first) err = launch_tile_depth(d_depth, frame->width, frame->height, v->d_tiled, v->stream);
     const CallInfo last{(int32_t)v->calls, (int32_t)((v->calls - 1) & 1)};
-    const hipError_t err = launch_proj_update(v->d, proj_cam(d_depth, frame), last, v->update_grid,
-                                              v->update_threads, v->stream);
+    if (err == hipSuccess)
+        err = launch_proj_update(v->d, proj_cam(v->d_tiled, frame), last, v->update_grid,
+                                 v->update_threads, v->stream);
     v->stage_end(2, e);
     if (err != hipSuccess) { v->fail("projective update launch", err); return 2; }
     return 0;
 }
 
+}  // namespace
+
+int32_t osn_tsdf_hip_allocate_depth(OsnTsdfHipVolume* v, const float* d_depth,
+                                    const OsnTsdfHipFrame* frame) {
+    return allocate_depth_tiled(v, d_depth, frame);
+}
+
+int32_t osn_tsdf_hip_update_depth(OsnTsdfHipVolume* v, const float* d_depth,
+                                  const OsnTsdfHipFrame* frame) {
+    return update_depth_tiled(v, d_depth, frame, true);
+}
+
 int32_t osn_tsdf_hip_integrate_depth(OsnTsdfHipVolume* v, const float* d_depth,
                                      const OsnTsdfHipFrame* frame) {
-    const int32_t rc = osn_tsdf_hip_allocate_depth(v, d_depth, frame);
+    const int32_t rc = allocate_depth_tiled(v, d_depth, frame);
     if (rc) return rc;
-    return osn_tsdf_hip_update_depth(v, d_depth, frame);
+    return update_depth_tiled(v, d_depth, frame, false);
 }
 
 int32_t osn_tsdf_hip_backproject(const float* d_depth, const OsnTsdfHipFrame* frame,
@@ -897,20 +937,20 @@ struct OsnTsdfHipBatch {
         AllocOpts o;
         o.lds_set = lds_set;
         o.grid = alloc_grid;
+        // With timing on, each stage's kernel records its own start and end
+        // (stage i: events 2i, 2i + 1).
         hipEvent_t e[6] = {};
-        auto mark = [&](int i) {
-            if (timing && hipEventCreate(&e[i]) == hipSuccess) hipEventRecord(e[i], stream);
-        };
-        mark(0);
-        hipError_t err = launch_alloc_batch(b, total_tiles, o, stream);
-        mark(1);
-        mark(2);
-        if (err == hipSuccess) err = launch_assign_batch(b, max_capacity, stream);
-        mark(3);
-        mark(4);
+        LaunchEvents ev[3];
+        if (timing) {
+            bool made = true;
+            for (int i = 0; i < 6 && made; ++i) made = hipEventCreate(&e[i]) == hipSuccess;
+            if (made)
+                for (int i = 0; i < 3; ++i) ev[i] = LaunchEvents{e[2 * i], e[2 * i + 1]};
+        }
+        hipError_t err = launch_alloc_batch(b, total_tiles, o, stream, ev[0]);
+        if (err == hipSuccess) err = launch_assign_batch(b, max_capacity, stream, ev[1]);
         if (err == hipSuccess)
-            err = launch_proj_update_batch(b, storage, update_grid, update_threads, stream);
-        mark(5);
+            err = launch_proj_update_batch(b, storage, update_grid, update_threads, stream, ev[2]);
         if (timing)
             for (int i = 0; i < 3; ++i) ev_pending.push_back({i, {e[2 * i], e[2 * i + 1]}});
         return err;
@@ -981,10 +1021,19 @@ int32_t osn_tsdf_hip_batch_set_schedule(OsnTsdfHipBatch* b, int32_t n_steps,
             if (!valid_frame(d_depths[i], &frames[i])) return 1;
             st.active = 1;
             st.src = depth_src(d_depths[i], &frames[i]);
-            st.cam = proj_cam(d_depths[i], &frames[i]);
+            st.cam = proj_cam(nullptr, &frames[i]);  // tiled copy, set below
             st.tiles = depth_units(frames[i].width, frames[i].height, kBatchDepthSub);
             if (st.tiles > tiles[s]) tiles[s] = st.tiles;
         }
+    // Each volume's tiled depth copy is sized for its largest frame before any
+    // pointer to it is recorded; allocate writes it and update reads it in the
+    // same step.
+    for (size_t i = 0; i < hs.size(); ++i)
+        if (hs[i].active && !b->vols[i % n]->grow_tiled(hs[i].src.W, hs[i].src.H)) return 2;
+    for (size_t i = 0; i < hs.size(); ++i) {
+        hs[i].src.tiled = b->vols[i % n]->d_tiled;
+        hs[i].cam.depth = b->vols[i % n]->d_tiled;
+    }
     std::vector<int32_t> prefix(n + 1, 0);
     for (int s = 0; s < n; ++s) prefix[s + 1] = prefix[s] + tiles[s];
     b->base_calls.assign(n, 0);

@@ -1,6 +1,8 @@
 // Host-side launch interface between the C ABI layer and the kernels.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "upd_params.hpp"
 
 namespace osn_hip {
@@ -11,6 +13,7 @@ struct PointSrc {
     const float* pos;      // 3*n interleaved, device
     int32_t n;
     const float* depth;    // W*H planar depth in metres, device; 0 = invalid
+    float* tiled;          // when set, the depth is also copied here in 8x4 tiles
     int32_t W, H;
     float fx, fy, cx, cy;
     float R[9];            // world-from-camera rotation, row-major (FP32)
@@ -100,18 +103,38 @@ struct ExtractArgs {
 constexpr int kSortTile = 4096;
 constexpr int kAssignMaxGrid = 256;
 
+// Optional events of one kernel launch. When set, the launch records them
+// itself (hipExtLaunchKernel): they mark its own dispatch start and end, not
+// separate stream commands around it, so their interval is the kernel's.
+struct LaunchEvents {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+
+template <typename... KArgs, typename... Args>
+inline void launch_kernel(void (*k)(KArgs...), dim3 grid, dim3 block, hipStream_t s,
+                          const LaunchEvents& ev, Args... args) {
+    if (ev.start)
+        hipExtLaunchKernelGGL(k, grid, block, 0, s, ev.start, ev.stop, 0, args...);
+    else
+        hipLaunchKernelGGL(k, grid, block, 0, s, args...);
+}
+
 hipError_t launch_alloc(const VolDev& v, const PointSrc& src, const AllocOpts& o,
                         const int32_t* coords, CallInfo call, hipStream_t s);
 hipError_t launch_alloc_batch(const BatchDev& b, int32_t total_tiles, const AllocOpts& o,
-                              hipStream_t s);
+                              hipStream_t s, const LaunchEvents& ev = {});
 hipError_t launch_backproject(const PointSrc& src, float* out, hipStream_t s);
+hipError_t launch_tile_depth(const float* depth, int32_t W, int32_t H, float* tiled, hipStream_t s);
 hipError_t launch_assign(const VolDev& v, CallInfo call, hipStream_t s);
-hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStream_t s);
+hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStream_t s,
+                               const LaunchEvents& ev = {});
 hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, hipStream_t s);
 hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call, int32_t grid,
                               int32_t block_threads, hipStream_t s);
 hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t grid,
-                                    int32_t block_threads, hipStream_t s);
+                                    int32_t block_threads, hipStream_t s,
+                                    const LaunchEvents& ev = {});
 hipError_t launch_step_advance(int32_t* step_ctr, hipStream_t s);
 hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipStream_t s);
 hipError_t launch_fill_table(HashEntry* table, uint32_t n, hipStream_t s);

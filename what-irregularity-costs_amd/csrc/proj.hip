@@ -7,7 +7,10 @@
 //
 //   p   = R_cw * ((v + 0.5) * voxel - c)                camera coordinates
 //   pix = floor(fx * p.x / p.z + cx + 0.5), floor(fy * p.y / p.z + cy + 0.5)
-//   d   = depth[pix]                    (skip unless p.z > 0, inside, d > 0)
+//   d   = depth[pix]                    (skip unless 2^-10 <= p.z <= 2^14,
+//                                        inside, d > 0)
+//                                       (read from a copy of the image in
+//                                       8x4-pixel tiles, one cache line each)
 //   sdf = d - p.z                       (skip if sdf < -trunc)
 //   x   = min(sdf * (1 / trunc), 1)
 //   t  <- (t * w + x) / (w + 1),  w <- min(w + 1, w_max)
@@ -20,8 +23,9 @@
 //   per block   (A0, B0, Z0) from d0 = R_cw * (block origin centre - c)
 //   per lane    off = (ly * F_y + lz * F_z) + lx0 * F_x  for its run of 4
 //               voxels along x starting at lx0 (0 or 4)
-//   per voxel   F = F0 + off, then F += F_x;  iz = 1 / Z; uf = A * iz, vf = B * iz
-// with every product rounded on its own (-ffp-contract=off).
+//   per voxel   F = F0 + off, then F += F_x;  uf = A * iz, vf = B * iz
+// with every product rounded on its own (-ffp-contract=off), where iz comes
+// from the run's batch inversion (run_reciprocals) of its four Z values.
 //
 // One wave per 8^3 block, each lane owning two runs of 4 consecutive voxels
 // along x (lz and lz + 4), so the lane's records are 16 contiguous bytes (FP16
@@ -94,6 +98,27 @@ __device__ inline void block_forms(const Params& k, int32_t bx, int32_t by, int3
     f0[2] = d0[2];
 }
 
+// Reciprocals of the four depths of a run by batch inversion: clamp each to
+// [kZMin, kZMax] (a voxel outside that range is rejected anyway), form the
+// running products, divide once (correctly rounded), and peel the reciprocals
+// off with the partial products. Six multiplies and one division instead of
+// four divisions; a few ulp from 1 / z, every step rounded on its own.
+__device__ inline void run_reciprocals(const float Z[4], float iz[4]) {
+    float z[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) z[j] = fminf(fmaxf(Z[j], kZMin), kZMax);
+    const float p1 = z[0] * z[1];
+    const float p2 = p1 * z[2];
+    const float p3 = p2 * z[3];
+    const float r3 = 1.0f / p3;
+    iz[3] = r3 * p2;
+    const float r2 = r3 * z[3];
+    iz[2] = r2 * p1;
+    const float r1 = r2 * z[2];
+    iz[1] = r1 * z[0];
+    iz[0] = r1 * z[1];
+}
+
 __device__ inline float avg_step(float t, float w, float x) {
     // (t * w + x) / (w + 1); the reciprocal's 1-ulp error is far inside the
     // storage rounding this feeds
@@ -116,34 +141,74 @@ __device__ inline u32x4 touch_entry(uint32_t part) {
                  (uint32_t)__builtin_amdgcn_readlane((int)part, 3)};
 }
 
-// One wave updates the whole block of touch entry e (uniform over the wave).
-// The update is latency-bound before it is bandwidth-bound, so everything a
-// block needs from memory is requested before any of it is used: the lane's
-// two voxel records, then the depth of all 8 of its voxels (pixels chosen
-// without branches: an out-of-image voxel reads pixel 0 and is then rejected).
-template <int STORE, typename Params>
-__device__ inline void update_block(const Params& k, const LaneRuns& L, uint32_t part, int lane) {
-    const u32x4 e = touch_entry(part);
-    int32_t bi = (int32_t)e.w;
-    if (bi == kIdxPending) bi = glob(k.table)[e.z].block_idx;  // inserted by this call
-    if (bi < 0) return;
+// A touched block, uniform over the wave: its pool index (negative: dropped,
+// nothing to update) and block coordinates.
+struct BlockRef {
+    int32_t bi;
     int32_t bx, by, bz;
-    unpack_key((int64_t)(((uint64_t)e.y << 32) | e.x), bx, by, bz);
-    const size_t v0 = (size_t)bi * kBlockVoxels + lane * 4;  // first voxel of run 0
+};
 
-    u32x4 h16[2];
-    f32x4 h32[2][2];
+template <typename Params>
+__device__ inline BlockRef resolve_block(const Params& k, uint32_t part) {
+    const u32x4 e = touch_entry(part);
+    BlockRef b;
+    b.bi = (int32_t)e.w;
+    if (b.bi == kIdxPending) b.bi = glob(k.table)[e.z].block_idx;  // inserted by this call
+    unpack_key((int64_t)(((uint64_t)e.y << 32) | e.x), b.bx, b.by, b.bz);
+    return b;
+}
+
+// The lane's voxel records of a block, read a whole block ahead of their
+// update: FP16 one 16-byte record per run, FP32 two. FP64 records are read
+// voxel by voxel after the gates.
+template <int STORE>
+struct Records {};
+template <>
+struct Records<kStoreF16> {
+    u32x4 h[2];
+};
+template <>
+struct Records<kStoreF32> {
+    f32x4 h[2][2];
+};
+
+// The loads are issued for every block, a dropped or absent one reading pool
+// block 0 (results unused), so the number of loads in flight is the same on
+// every path and the wait for the gathers can leave these outstanding.
+template <int STORE, typename Params>
+__device__ inline Records<STORE> load_records(const Params& k, const BlockRef& b, int lane) {
+    Records<STORE> R{};
+    const size_t v0 = (size_t)(b.bi < 0 ? 0 : b.bi) * kBlockVoxels + lane * 4;
     if constexpr (STORE == kStoreF16) {
         const u32x4 OSN_GLOBAL* rec = reinterpret_cast<const u32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 4);
-        h16[0] = rec[0];
-        h16[1] = rec[64];  // run 1 starts 256 voxels (1 KiB) later
+        R.h[0] = rec[0];
+        R.h[1] = rec[64];  // run 1 starts 256 voxels (1 KiB) later
     } else if constexpr (STORE == kStoreF32) {
         const f32x4 OSN_GLOBAL* rec = reinterpret_cast<const f32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 8);
-        h32[0][0] = rec[0];
-        h32[0][1] = rec[1];
-        h32[1][0] = rec[128];
-        h32[1][1] = rec[129];
+        R.h[0][0] = rec[0];
+        R.h[0][1] = rec[1];
+        R.h[1][0] = rec[128];
+        R.h[1][1] = rec[129];
     }
+    return R;
+}
+
+// One wave updates the whole block b (uniform over the wave) from its records
+// R, requested one block earlier. The update is latency-bound before it is
+// bandwidth-bound, so loads are kept in flight across blocks: the depth of all
+// 8 of the lane's voxels is gathered first (pixels chosen without branches: an
+// out-of-image voxel reads pixel 0 and is then rejected), then ahead() runs,
+// which resolves the next block and requests its records, and only then does
+// the wave wait, for the gathers alone (loads complete in issue order).
+template <int STORE, typename Params, typename Ahead>
+__device__ inline void update_block(const Params& k, const LaneRuns& L, const BlockRef& b,
+                                    const Records<STORE>& R, int lane, Ahead&& ahead) {
+    if (b.bi < 0) {
+        ahead();
+        return;
+    }
+    const int32_t bx = b.bx, by = b.by, bz = b.bz;
+    const size_t v0 = (size_t)b.bi * kBlockVoxels + lane * 4;  // first voxel of run 0
 
     float f0[3];
     block_forms(k, bx, by, bz, f0);
@@ -152,34 +217,35 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, uint32_t
     bool inb[8];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-        float A = f0[0] + L.off[r][0], B = f0[1] + L.off[r][1], Z = f0[2] + L.off[r][2];
+        float A[4], B[4], Z[4], iz[4];
+        A[0] = f0[0] + L.off[r][0];
+        B[0] = f0[1] + L.off[r][1];
+        Z[0] = f0[2] + L.off[r][2];
+#pragma unroll
+        for (int j = 1; j < 4; ++j) {
+            A[j] = A[j - 1] + k.g[0][0];
+            B[j] = B[j - 1] + k.g[1][0];
+            Z[j] = Z[j - 1] + k.g[2][0];
+        }
+        run_reciprocals(Z, iz);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float iz = 1.0f / Z;
-            const float uf = A * iz;
-            const float vf = B * iz;
-            const bool in = (Z > 0.0f) & (uf >= 0.0f) & (vf >= 0.0f) & (uf < (float)k.W) &
-                            (vf < (float)k.H);
-            // uf, vf >= 0: truncation is floor; W, H < 2^24, so the index fits 24x24 bits
+            const float uf = A[j] * iz[j];
+            const float vf = B[j] * iz[j];
+            const bool in = (Z[j] >= kZMin) & (Z[j] <= kZMax) & (uf >= 0.0f) & (vf >= 0.0f) &
+                            (uf < (float)k.W) & (vf < (float)k.H);
+            // uf, vf >= 0: truncation is floor; the depth is read from the
+            // 8x4-tiled copy of the image
             const uint32_t u = in ? (uint32_t)uf : 0u, w = in ? (uint32_t)vf : 0u;
-            pix[r * 4 + j] = __umul24(w, (uint32_t)k.W) + u;
+            pix[r * 4 + j] = tiled_index(u, w, (uint32_t)k.tile_k);
             inb[r * 4 + j] = in;
-            Zv[r * 4 + j] = Z;
-            A = A + k.g[0][0];
-            B = B + k.g[1][0];
-            Z = Z + k.g[2][0];
+            Zv[r * 4 + j] = Z[j];
         }
     }
     float d[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) d[i] = glob(k.depth)[pix[i]];
-    // The records are read before the gates decide whether they change; left
-    // alone, the compiler would move a record load behind the gates (into the
-    // store branch) and expose its latency. This empty statement, which emits
-    // no instruction, needs them here, after the gathers were issued.
-    if constexpr (STORE == kStoreF16) asm volatile("" ::"v"(h16[0]), "v"(h16[1]));
-    if constexpr (STORE == kStoreF32)
-        asm volatile("" ::"v"(h32[0][0]), "v"(h32[0][1]), "v"(h32[1][0]), "v"(h32[1][1]));
+    ahead();
     bool ok[8];
     float sdf[8];
 #pragma unroll
@@ -194,7 +260,7 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, uint32_t
         const float* sd = sdf + r * 4;
         if constexpr (STORE == kStoreF16) {
             if (o[0] | o[1] | o[2] | o[3]) {
-                uint32_t word[4] = {h16[r].x, h16[r].y, h16[r].z, h16[r].w};
+                uint32_t word[4] = {R.h[r].x, R.h[r].y, R.h[r].z, R.h[r].w};
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const float x = fminf(sd[j] * k.inv_trunc, 1.0f);
@@ -211,8 +277,8 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, uint32_t
             }
         } else if constexpr (STORE == kStoreF32) {
             if (o[0] | o[1] | o[2] | o[3]) {
-                float tv[4] = {h32[r][0].x, h32[r][0].z, h32[r][1].x, h32[r][1].z};
-                float wv[4] = {h32[r][0].y, h32[r][0].w, h32[r][1].y, h32[r][1].w};
+                float tv[4] = {R.h[r][0].x, R.h[r][0].z, R.h[r][1].x, R.h[r][1].z};
+                float wv[4] = {R.h[r][0].y, R.h[r][0].w, R.h[r][1].y, R.h[r][1].w};
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const float x = fminf(sd[j] * k.inv_trunc, 1.0f);
@@ -259,13 +325,30 @@ __global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, Call
     const LaneRuns L = lane_runs(k, lane);
     const int stride = gridDim.x * waves;
     int item = blockIdx.x * waves + wave_id();
-    uint32_t e = item < k.n ? load_touch(k, item, lane) : 0u;
-    while (item < k.n) {  // the next block's entry is requested a block ahead
-        const int next = item + stride;
-        const uint32_t en = next < k.n ? load_touch(k, next, lane) : 0u;
-        update_block<STORE>(k, L, e, lane);
-        e = en;
-        item = next;
+    if (item >= k.n) return;
+    // Pipeline: while block i is updated, the entry of block i + 2 and the
+    // records of block i + 1 are in flight. A pass updates cur from `use` while
+    // it fills `fill` and `ent_fill`; passes alternate the two register sets,
+    // so nothing in flight is ever copied (a copy would wait for the load).
+    BlockRef cur = resolve_block(k, load_touch(k, item, lane));
+    Records<STORE> ra = load_records<STORE>(k, cur, lane), rb{};
+    int next = item + stride;
+    uint32_t ea = load_touch(k, next < k.n ? next : 0, lane), eb = 0u;
+    auto pass = [&](const Records<STORE>& use, Records<STORE>& fill, uint32_t ent,
+                    uint32_t& ent_fill) {
+        const int next2 = next + stride;
+        BlockRef nb{-1, 0, 0, 0};
+        update_block<STORE>(k, L, cur, use, lane, [&] {
+            if (next < k.n) nb = resolve_block(k, ent);
+            ent_fill = load_touch(k, next2 < k.n ? next2 : 0, lane);  // issued on every path
+            fill = load_records<STORE>(k, nb, lane);
+        });
+        cur = nb;
+        const bool more = next < k.n;
+        next = next2;
+        return more;
+    };
+    while (pass(ra, rb, ea, eb) && pass(rb, ra, eb, ea)) {
     }
 }
 
@@ -296,24 +379,43 @@ __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
     const int lo = (int)((long long)n * xcd / kXcds);
     const int hi = (int)((long long)n * (xcd + 1) / kXcds);
     const int stride = per_xcd * waves;
-    int item = lo + (int)(blockIdx.x / kXcds) * waves + wave_id();
-    int sc = 0, lanes_sc = -1;
-    while (item < hi && s_prefix[sc + 1] <= item) ++sc;
-    uint32_t e = item < hi ? load_touch(P[sc], item - s_prefix[sc], lane) : 0u;
+    const int item = lo + (int)(blockIdx.x / kXcds) * waves + wave_id();
+    if (item >= hi) return;
+    // Pipeline as in k_proj_update. Items ascend, so the scene of the next
+    // items only moves forward.
+    int sc = 0;
+    while (s_prefix[sc + 1] <= item) ++sc;
+    BlockRef cur = resolve_block(P[sc], load_touch(P[sc], item - s_prefix[sc], lane));
+    Records<STORE> ra = load_records<STORE>(P[sc], cur, lane), rb{};
+    int next = item + stride, scn = sc;
+    while (next < hi && s_prefix[scn + 1] <= next) ++scn;
+    // entry 0 of this scene stands in for an absent next entry (never used)
+    uint32_t ea = load_touch(P[scn], next < hi ? next - s_prefix[scn] : 0, lane), eb = 0u;
+    int lanes_sc = -1;
     LaneRuns L{};
-    while (item < hi) {  // the next block's entry is requested a block ahead
-        const int next = item + stride;
-        int scn = sc;  // items ascend, so the scene only moves forward
-        while (next < hi && s_prefix[scn + 1] <= next) ++scn;
-        const uint32_t en = next < hi ? load_touch(P[scn], next - s_prefix[scn], lane) : 0u;
+    auto pass = [&](const Records<STORE>& use, Records<STORE>& fill, uint32_t ent,
+                    uint32_t& ent_fill) {
+        const int next2 = next + stride;
+        int scn2 = scn;
+        while (next2 < hi && s_prefix[scn2 + 1] <= next2) ++scn2;
         if (sc != lanes_sc) {  // the lane offsets depend on the scene's pose
             L = lane_runs(P[sc], lane);
             lanes_sc = sc;
         }
-        update_block<STORE>(P[sc], L, e, lane);
-        e = en;
-        item = next;
+        BlockRef nb{-1, 0, 0, 0};
+        update_block<STORE>(P[sc], L, cur, use, lane, [&] {
+            if (next < hi) nb = resolve_block(P[scn], ent);
+            ent_fill = load_touch(P[scn2], next2 < hi ? next2 - s_prefix[scn2] : 0, lane);
+            fill = load_records<STORE>(P[scn], nb, lane);
+        });
+        cur = nb;
         sc = scn;
+        scn = scn2;
+        const bool more = next < hi;
+        next = next2;
+        return more;
+    };
+    while (pass(ra, rb, ea, eb) && pass(rb, ra, eb, ea)) {
     }
 }
 
@@ -335,7 +437,8 @@ hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call
 }
 
 hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t grid,
-                                    int32_t block_threads, hipStream_t s) {
+                                    int32_t block_threads, hipStream_t s,
+                                    const LaunchEvents& ev) {
     if (b.n_scenes > kMaxBatchScenes) return hipErrorInvalidValue;
     if (grid <= 0) grid = 4096;
     grid = (grid + kXcds - 1) / kXcds * kXcds;
@@ -343,9 +446,9 @@ hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t 
         return hipErrorInvalidValue;
     if (block_threads < b.n_scenes) block_threads = 512;  // one thread per scene gathers
     switch (storage) {
-        case kStoreF16: k_proj_update_batch<kStoreF16><<<grid, block_threads, 0, s>>>(b); break;
-        case kStoreF32: k_proj_update_batch<kStoreF32><<<grid, block_threads, 0, s>>>(b); break;
-        case kStoreF64: k_proj_update_batch<kStoreF64><<<grid, block_threads, 0, s>>>(b); break;
+        case kStoreF16: launch_kernel(k_proj_update_batch<kStoreF16>, dim3(grid), dim3(block_threads), s, ev, b); break;
+        case kStoreF32: launch_kernel(k_proj_update_batch<kStoreF32>, dim3(grid), dim3(block_threads), s, ev, b); break;
+        case kStoreF64: launch_kernel(k_proj_update_batch<kStoreF64>, dim3(grid), dim3(block_threads), s, ev, b); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

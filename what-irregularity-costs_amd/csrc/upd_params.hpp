@@ -7,6 +7,30 @@
 
 namespace osn_hip {
 
+// The projective update reads depth from a copy in 8x4-pixel tiles: tile
+// (tx, ty) holds 32 floats, row-major within the tile, at
+// (ty * ceil(W / 8) + tx) * 32, so one tile is one 128-byte cache line and the
+// pixels a wave gathers for a block fall in about half as many lines as in
+// row-major order. The copy covers the image rounded up to whole tiles (padding
+// 0). Frames are limited to kMaxFrameDim per side, so every index fits 32 bits
+// and the index arithmetic fits 24-bit multiplies.
+constexpr int32_t kMaxFrameDim = 1 << 14;
+__host__ __device__ inline int32_t tiled_width(int32_t W) { return (W + 7) & ~7; }
+__host__ __device__ inline int32_t tiled_height(int32_t H) { return (H + 3) & ~3; }
+__host__ __device__ inline int32_t tiled_stride_k(int32_t W) { return 4 * tiled_width(W) - 32; }
+// Index of pixel (u, w) with k = tiled_stride_k(W): for u = 8a + b, w = 4c + d
+// it is c * 4 * tiled_width + 32 a + 8 d + b.
+__device__ inline uint32_t tiled_index(uint32_t u, uint32_t w, uint32_t k) {
+    return __umul24(w >> 2, k) + (w << 3) + (__umul24(u >> 3, 24u) + u);
+}
+
+// Camera depth range of an observable voxel centre, metres. Within it the
+// reciprocals of a run of four depths come from one division (batch
+// inversion: the product's reciprocal times partial products), never
+// overflowing or underflowing.
+constexpr float kZMin = 1.0f / 1024.0f;
+constexpr float kZMax = 16384.0f;
+
 // Camera for the projective update: world -> camera rotation and the camera
 // centre, FP32, plus intrinsics and the depth image.
 struct ProjCam {
@@ -14,7 +38,7 @@ struct ProjCam {
     float c[3];
     float fx, fy, cx, cy;
     int32_t W, H;
-    const float* depth;
+    const float* depth;  // the depth image in 8x4-pixel tiles
     double Rcw64[9];   // FP64 pose, read only by the FP64 oracle kernel
     double c64[3];
 };
@@ -24,7 +48,7 @@ struct UpdParams {
     const int32_t* block_coord;
     char* vox;
     const TouchEntry* touched;
-    const float* depth;
+    const float* depth;  // 8x4-pixel tiles
     float Rcw[9];
     float c[3];
     float g[3][3];      // g[F][j]: step of form F (A, B, Z) along voxel axis j
@@ -32,7 +56,7 @@ struct UpdParams {
     int32_t W, H;
     float voxel, trunc, inv_trunc, wmax;
     int32_t n;          // touched blocks
-    int32_t pad;
+    int32_t tile_k;     // tiled_stride_k(W)
     double Rcw64[9];    // FP64 oracle kernel only
     double c64[3];
 };
@@ -67,7 +91,7 @@ __device__ inline UpdParams make_params(const VolDev& v, const ProjCam& k, int p
     p.inv_trunc = 1.0f / v.trunc;
     p.wmax = (float)v.w_max;
     p.n = v.ctr->lists[parity].touched_count;
-    p.pad = 0;
+    p.tile_k = tiled_stride_k(k.W);
     return p;
 }
 
