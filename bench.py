@@ -40,6 +40,18 @@ sys.path.insert(0, os.path.join(ROOT, "what-irregularity-costs_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per update launch from FETCH_SIZE and WRITE_SIZE counter runs of
+# `bench.py --profile-only` (rocprofv3 --pmc, one counter per run), each
+# scaled by what the counter reads for a 256 MiB device copy (FETCH x 2.0,
+# WRITE x 1.0). Those launches are steps 0..29 of the same 16 scenes, so their
+# algorithmic bytes are given beside them; counters cannot run inside this
+# timed benchmark.
+UPDATE_TRAFFIC = {"hbm_bytes_per_launch": 397.9e6, "read": 241.1e6, "write": 156.8e6,
+                  "algorithmic_bytes_per_launch": 432.9e6,
+                  "launches": "k_proj_update_batch<fp16>, 16 scenes, steps 0..29, "
+                              "100.6k touched blocks per launch",
+                  "summary": "profiles/k_proj_update_batch__precise16_640x480.md"}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -154,7 +166,7 @@ def run_batched_instrumented(H, cfg, storage, scenes, warmup, steps, bytes_per_v
     touched = []
     for _ in range(warmup + steps):
         b.run(1)
-        touched.append(sum(v.touched_count for v in vols))  # synchronises
+        touched.append(sum(v.touched_count for v in vols))  # waits for the batch stream
     W, Hh = cfg["width"], cfg["height"]
     tb_all = np.array(touched, np.float64)
     tb = tb_all[warmup:]
@@ -462,7 +474,9 @@ def main():
         "chain_with_extraction": chain,
         "roofline": {"bound": "hbm", "achieved": round(float(achieved16), 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(float(achieved16 / HBM_PEAK_GBS), 4), "traffic": None,
+                     "frac": round(float(achieved16 / HBM_PEAK_GBS), 4),
+                     "traffic": UPDATE_TRAFFIC["hbm_bytes_per_launch"],
+                     "traffic_measured_on": UPDATE_TRAFFIC,
                      "kernel": "k_proj_update_batch<fp16>",
                      "avg_launch_ms": round(float(upd_ms16), 5),
                      "bytes_per_launch": round(float(bytes16.mean()), 0)},

@@ -115,6 +115,30 @@ def test_graph_replay_matches_eager_every_step(H, S):
         assert same(state(v1), state(v2))
 
 
+def test_member_reads_wait_for_the_batch(H, S):
+    """Counters and voxels of a batched volume read straight after an
+    asynchronous batch step (no explicit synchronisation) are that step's: the
+    reads wait for the batch's own stream, not only the volume's."""
+    n_steps = 5
+    depths, frames = make_inputs(H, S, n_steps, seed0=110)
+    ref = volumes(H, 2)
+    b0 = H.Batch(ref)
+    assert b0.set_schedule(depths, frames) == 0
+    want = []
+    for k in range(n_steps):
+        assert b0.run(1) == 0
+        b0.synchronize()
+        want.append([(v.touched_count, v.block_count) for v in ref])
+    vs = volumes(H, 2)
+    b = H.Batch(vs)
+    assert b.set_schedule(depths, frames) == 0
+    for k in range(n_steps):
+        assert b.run(1) == 0
+        assert [(v.touched_count, v.block_count) for v in vs] == want[k], f"step {k}"
+    for v1, v2 in zip(ref, vs):
+        assert same(state(v1), state(v2))
+
+
 def test_batch_tuning_keeps_results(H, S):
     n_steps = 4
     depths, frames = make_inputs(H, S, n_steps, seed0=90)

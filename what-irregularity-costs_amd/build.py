@@ -59,6 +59,7 @@ def build(verbose: bool = False) -> str:
     if verbose:
         print(f"built {LIB}")
     build_harness(verbose)
+    build_ceiling(verbose)
     return LIB
 
 
@@ -79,6 +80,24 @@ def build_harness(verbose: bool = False) -> str:
     if verbose:
         print(f"built {HARNESS}")
     return HARNESS
+
+
+CEILING_SRC = os.path.join(HERE, "harness", "rmw_ceiling.hip")
+CEILING = os.path.join(HERE, "rmw_ceiling")
+
+
+def build_ceiling(verbose: bool = False) -> str:
+    """The memory-ceiling probe of the update's access pattern (standalone)."""
+    tmp = CEILING + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-Wall", CEILING_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"rmw_ceiling build failed:\n{r.stderr}")
+    os.replace(tmp, CEILING)
+    if verbose:
+        print(f"built {CEILING}")
+    return CEILING
 
 
 if __name__ == "__main__":

@@ -71,6 +71,7 @@ struct OsnTsdfHipVolume {
     int32_t alloc_grid = 0;
     int64_t calls = 0;  // allocate calls since create/reset: call c stamps c + 1, uses list set c & 1
     bool in_batch = false;
+    hipStream_t batch_stream = nullptr;  // the batch's stream while in a batch
     // extraction scratch, grown on demand
     int32_t* ex_vcount = nullptr;
     int32_t* ex_voff = nullptr;
@@ -100,6 +101,13 @@ struct OsnTsdfHipVolume {
     bool grow_extract(int32_t blocks);
     bool grow_stage(int64_t verts, bool rgb);
     bool grow_tiled(int32_t W, int32_t H);
+    // Wait for every stream that writes this volume: its own and, while it
+    // belongs to a batch, the batch's (a non-blocking stream of its own).
+    hipError_t sync() const {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e == hipSuccess && batch_stream) e = hipStreamSynchronize(batch_stream);
+        return e;
+    }
 };
 
 bool OsnTsdfHipVolume::fail(const char* what, hipError_t e) {
@@ -361,7 +369,7 @@ bool valid_frame(const float* d_depth, const OsnTsdfHipFrame* f) {
 // wait, so the pool order is incomplete and the volume is marked invalid.
 template <typename T>
 int32_t read_counter(const OsnTsdfHipVolume* v, size_t offset, T* out) {
-    if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    if (v->sync() != hipSuccess) return 2;
     int32_t head[2];
     if (hipMemcpy(head, v->d.ctr, sizeof(head), hipMemcpyDeviceToHost) != hipSuccess) return 2;
     if (head[1] != 0) {
@@ -610,6 +618,8 @@ int32_t osn_tsdf_hip_extract_mesh_device(OsnTsdfHipVolume* v, float* d_posnor, u
                                          int32_t* d_count) {
     if (!v || !d_posnor || vert_cap <= 0 || !d_count) return 1;
     if (!v->ok) return 2;
+    // a batch writes this volume on its own stream: let that work finish first
+    if (v->batch_stream && hipStreamSynchronize(v->batch_stream) != hipSuccess) return 2;
     if (!v->grow_extract(v->d.pool_capacity)) { v->ok = false; return 2; }
     ExtractArgs a{};
     a.min_weight = min_weight;
@@ -706,7 +716,7 @@ int32_t osn_tsdf_hip_touched_count(const OsnTsdfHipVolume* v) {
 }
 
 void osn_tsdf_hip_synchronize(const OsnTsdfHipVolume* v) {
-    if (v && v->stream) hipStreamSynchronize(v->stream);
+    if (v && v->stream) v->sync();
 }
 
 int32_t osn_tsdf_hip_block_keys(const OsnTsdfHipVolume* v, int64_t* host_keys, int32_t cap) {
@@ -725,7 +735,7 @@ int32_t osn_tsdf_hip_block_keys(const OsnTsdfHipVolume* v, int64_t* host_keys, i
 int32_t osn_tsdf_hip_read_voxels(const OsnTsdfHipVolume* v, int32_t first, int32_t n,
                                  float* host_tsdf, float* host_weight) {
     if (!v || !v->ok || first < 0 || n <= 0 || first + n > v->d.pool_capacity) return 1;
-    if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    if (v->sync() != hipSuccess) return 2;
     const size_t o = (size_t)first * kBlockVoxels;
     const size_t cnt = (size_t)n * kBlockVoxels;
     if (v->d.storage == kStoreSums) {
@@ -777,7 +787,7 @@ int32_t osn_tsdf_hip_read_voxels_f64(const OsnTsdfHipVolume* v, int32_t first, i
                                      double* host_tsdf, double* host_weight) {
     if (!v || !v->ok || first < 0 || n <= 0 || first + n > v->d.pool_capacity) return 1;
     if (v->d.storage != kStoreF64) return 3;
-    if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    if (v->sync() != hipSuccess) return 2;
     const size_t cnt = (size_t)n * kBlockVoxels;
     std::vector<double> raw(cnt * 2);
     if (hipMemcpy(raw.data(), static_cast<double*>(v->d.vox) + (size_t)first * kBlockVoxels * 2,
@@ -794,7 +804,7 @@ int32_t osn_tsdf_hip_probe_stats(const OsnTsdfHipVolume* v, uint64_t* hist64,
                                  uint64_t* lane_steps, uint64_t* wave_steps, uint64_t* lookups) {
     if (!v || !v->ok) return 1;
     Counters c;
-    if (hipStreamSynchronize(v->stream) != hipSuccess) return 2;
+    if (v->sync() != hipSuccess) return 2;
     if (hipMemcpy(&c, v->d.ctr, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) return 2;
     if (hist64) for (int i = 0; i < kProbeHistBins; ++i) hist64[i] = c.probe_hist[i];
     if (lane_steps) *lane_steps = c.lane_steps;
@@ -963,7 +973,10 @@ struct OsnTsdfHipBatch {
                         (void*)d_steps, (void*)d_upd})
             if (p) hipFree(p);
         if (stream) hipStreamDestroy(stream);
-        for (auto* v : vols) v->in_batch = false;
+        for (auto* v : vols) {
+            v->in_batch = false;
+            v->batch_stream = nullptr;
+        }
     }
 };
 
@@ -997,7 +1010,10 @@ OsnTsdfHipBatch* osn_tsdf_hip_batch_create(OsnTsdfHipVolume* const* volumes, int
               hipMalloc(&b->d_upd, sizeof(UpdParams) * n) == hipSuccess &&
               hipMemcpy(b->d_vols, hv.data(), sizeof(VolDev) * n, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) { delete b; return nullptr; }
-    for (auto* v : b->vols) v->in_batch = true;
+    for (auto* v : b->vols) {
+        v->in_batch = true;
+        v->batch_stream = b->stream;
+    }
     return b;
 }
 
