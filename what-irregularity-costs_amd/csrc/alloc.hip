@@ -26,7 +26,11 @@ namespace osn_hip {
 namespace {
 
 constexpr int kAllocThreads = 256;  // 4 waves; an 8x8-pixel quarter tile each in depth mode
-constexpr int kKeysPerLane = 8;     // distinct keys a lane buffers before a flush
+constexpr int kKeysPerLane = 4;     // distinct keys a lane buffers before a flush
+// The walk's registers bound its occupancy (the probes and atomics want many
+// waves in flight): at 6 waves per SIMD it fits 80 VGPRs with a few spilled
+// bytes; 7 spills more than it gains, 5 leaves latency exposed.
+constexpr int kAllocWavesPerSimd = 6;
 constexpr int kWgList = 512;        // LDS touched-list capacity per workgroup
 constexpr int kWgNew = 256;         // LDS new-key-list capacity per workgroup
 constexpr int kMaxSet = 1024;       // LDS pre-hash capacity
@@ -577,7 +581,7 @@ __device__ inline void reset_other_parity(Counters* ctr, int parity) {
 }
 
 template <int MODE, bool TOUCH, int SRC>
-__global__ __launch_bounds__(kAllocThreads) void k_alloc(VolDev v, PointSrc src,
+__global__ __launch_bounds__(kAllocThreads, kAllocWavesPerSimd) void k_alloc(VolDev v, PointSrc src,
                                                          const int32_t* coords, int32_t set_size,
                                                          CallInfo call, int32_t n_tiles) {
     __shared__ WgShared s;
@@ -598,7 +602,7 @@ __global__ __launch_bounds__(kAllocThreads) void k_alloc(VolDev v, PointSrc src,
 // one work list (scene s owns [tile_prefix[s], tile_prefix[s + 1]), read once
 // into LDS); each scene keeps its own table and lists, flushed when the
 // workgroup moves on to another scene.
-__global__ __launch_bounds__(kAllocThreads) void k_alloc_batch(BatchDev b, int32_t set_size,
+__global__ __launch_bounds__(kAllocThreads, kAllocWavesPerSimd) void k_alloc_batch(BatchDev b, int32_t set_size,
                                                                int32_t total_tiles) {
     __shared__ WgShared s;
     __shared__ int32_t s_prefix[kMaxBatchScenes + 1];
