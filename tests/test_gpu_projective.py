@@ -83,9 +83,13 @@ def test_backprojection_bit_exact(H, S):
         assert got.shape == want.shape and np.array_equal(got, want)
 
 
-def test_depth_allocation_key_sets_bit_exact(H, S):
+# trunc 8 cm = 4 voxels: the walk takes its bisection shortcut on nearly every
+# ray. trunc 7 cm = 3.5 voxels: the band still has 4 samples a side, but two
+# of them can be culled, so every wave walks sample by sample.
+@pytest.mark.parametrize("trunc", [0.08, 0.07])
+def test_depth_allocation_key_sets_bit_exact(H, S, trunc):
     """Small config: 20 frames 640x480, 2 cm voxels, table 2^18."""
-    cfg = S.SMALL
+    cfg = dict(S.SMALL, trunc=trunc)
     _, fr = frames_for(H, S, cfg, 5, cfg["frames"])
     gv = H.HipVolume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 0.0, storage=H.STORE_F32)
     ov = B.Volume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 0.0)

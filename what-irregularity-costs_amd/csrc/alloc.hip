@@ -342,6 +342,42 @@ __device__ void flush_wave(const TileCtx& c, LaneKeys& lk) {
     if (nd > 0) process_keys<MODE, TOUCH>(c, __lane_id() < nd, mine);
 }
 
+// The wave's window cells and its lanes' buffered keys in as few
+// process_keys calls as possible (usually one): lane i takes window cell i
+// when it is marked, and each distinct buffered key (a key outside the window)
+// goes to a lane whose cell is empty. A window holds a few blocks of a wave's
+// 64 neighbouring pixels, so empty lanes abound; a second call follows only
+// when they run out. Buffered keys are deduplicated across lanes first-live-
+// key first, as in flush_wave, and lie outside the window, so no key reaches
+// two lanes of one call.
+template <int MODE, bool TOUCH>
+__device__ void process_window_and_buffers(const TileCtx& c, unsigned long long cells, int32_t ox,
+                                           int32_t oy, int32_t oz, LaneKeys& lk) {
+    const int lane = __lane_id();
+    bool has = (cells >> lane) & 1ull;
+    int64_t key = pack_coord(ox + (lane & 3), oy + ((lane >> 2) & 3), oz + (lane >> 4));
+    unsigned long long free_lanes = ~cells;  // wave-uniform
+    for (;;) {
+        const unsigned long long m = __ballot(lk.valid != 0);
+        if (m == 0) break;
+        if (free_lanes == 0) {
+            process_keys<MODE, TOUCH>(c, has, key);
+            has = false;
+            free_lanes = ~0ull;
+        }
+        const int leader = __ffsll((long long)m) - 1;
+        const int64_t k = readlane64(lk.first(), leader);
+        lk.remove(k);
+        const int dst = __ffsll((long long)free_lanes) - 1;
+        free_lanes &= free_lanes - 1;
+        if (lane == dst) {
+            has = true;
+            key = k;
+        }
+    }
+    process_keys<MODE, TOUCH>(c, has, key);
+}
+
 // Band walk (volume.cpp:158-191) state of one point.
 struct Ray {
     float px, py, pz;  // the point
@@ -376,32 +412,123 @@ __device__ inline bool make_ray(const VolDev& v, const PointSrc& src, float px, 
     return !(src.radius_sq > 0.0f && d2 > src.radius_sq) && (dist > 1e-6f);
 }
 
+// Voxel coordinate of sample s along one axis: floor((p + u * t) * (1 / voxel))
+// with t = s * voxel, every operation rounded on its own.
+__device__ inline int32_t sample_voxel(float p, float u, float t, float inv_voxel) {
+    return (int32_t)floorf((p + u * t) * inv_voxel);
+}
+
+// B of sample s (see make_ray): t = s * voxel with voxel > 0, so B = t for s >= 0.
+__device__ inline float sample_b(const Ray& r, int s, float t) {
+    return s >= 0 ? t : (r.dist + t >= 0.0f ? t : -t);
+}
+
+// The reference's cull of a sample in voxel (vx, vy, vz) whose B >= thr.
+__device__ inline bool sample_culled(const VolDev& v, const PointSrc& src, const Ray& r, int32_t vx,
+                                     int32_t vy, int32_t vz) {
+    const float cx = ((float)vx + 0.5f) * v.voxel;
+    const float cy = ((float)vy + 0.5f) * v.voxel;
+    const float cz = ((float)vz + 0.5f) * v.voxel;
+    const float ex = cx - src.cam[0], ey = cy - src.cam[1], ez = cz - src.cam[2];
+    const float sdf = r.dist - sqrtf(ex * ex + ey * ey + ez * ez);
+    return sdf < -v.trunc;
+}
+
+// Block (bx, by, bz) goes to the wave's 4x4x4-block window (cells) or, when
+// outside it, to the lane buffer.
+__device__ inline void mark_block(int32_t bx, int32_t by, int32_t bz, int32_t ox, int32_t oy,
+                                  int32_t oz, unsigned long long& cells, LaneKeys& lk) {
+    const uint32_t dx = uint32_t(bx - ox), dy = uint32_t(by - oy), dz = uint32_t(bz - oz);
+    if ((dx | dy | dz) < 4u) cells |= 1ull << (dx | (dy << 2) | (dz << 4));
+    else lk.push(pack_coord(bx, by, bz));
+}
+
 // Sample s of a live lane's band walk: the voxel (p + u * t) / voxel, unless
-// culled, marks its block, in the wave's 4x4x4-block window (cells) or in the
-// lane buffer.
+// culled, marks its block.
 __device__ inline void mark_sample(const VolDev& v, const PointSrc& src, const Ray& r, int s,
                                    float t, int32_t ox, int32_t oy, int32_t oz,
                                    unsigned long long& cells, LaneKeys& lk) {
-    const int32_t vx = (int32_t)floorf((r.px + r.ux * t) * v.inv_voxel);
-    const int32_t vy = (int32_t)floorf((r.py + r.uy * t) * v.inv_voxel);
-    const int32_t vz = (int32_t)floorf((r.pz + r.uz * t) * v.inv_voxel);
-    // t = s * voxel with voxel > 0, so B = t for s >= 0
-    const float B = s >= 0 ? t : (r.dist + t >= 0.0f ? t : -t);
-    bool culled = false;
-    if (B >= r.thr) {
-        const float cx = ((float)vx + 0.5f) * v.voxel;
-        const float cy = ((float)vy + 0.5f) * v.voxel;
-        const float cz = ((float)vz + 0.5f) * v.voxel;
-        const float ex = cx - src.cam[0], ey = cy - src.cam[1], ez = cz - src.cam[2];
-        const float sdf = r.dist - sqrtf(ex * ex + ey * ey + ez * ez);
-        culled = sdf < -v.trunc;
+    const int32_t vx = sample_voxel(r.px, r.ux, t, v.inv_voxel);
+    const int32_t vy = sample_voxel(r.py, r.uy, t, v.inv_voxel);
+    const int32_t vz = sample_voxel(r.pz, r.uz, t, v.inv_voxel);
+    const bool culled = sample_b(r, s, t) >= r.thr && sample_culled(v, src, r, vx, vy, vz);
+    if (!culled) mark_block(floor_div8(vx), floor_div8(vy), floor_div8(vz), ox, oy, oz, cells, lk);
+}
+
+// The blocks of samples -S..S of a live lane, exactly as marking every sample
+// would find them, from a few samples. Each voxel coordinate of sample s is a
+// monotone function of s (t = s * voxel grows with s, and every FP32
+// operation of sample_voxel is monotone), so each block coordinate is too.
+// In the usual case only sample S can be culled (no other sample reaches
+// B >= thr), the non-culled samples are a range -S..b, and the blocks of its
+// ends differ by at most one along every axis. Then each axis changes block
+// once, at a sample c found by bisection, and the range's blocks are those at
+// -S and at each c: at most 4, so the lane buffer cannot overflow. Returns
+// false, marking nothing, in any other case; the caller then marks every
+// sample.
+template <int S>
+__device__ inline bool mark_band(const VolDev& v, const PointSrc& src, const Ray& r, int32_t ox,
+                                 int32_t oy, int32_t oz, unsigned long long& cells,
+                                 LaneKeys& lk) {
+    bool other_b = false;  // a sample other than S with B >= thr
+#pragma unroll
+    for (int s = -S; s < S; ++s) other_b |= sample_b(r, s, (float)s * v.voxel) >= r.thr;
+    const float ta = (float)(-S) * v.voxel, te = (float)S * v.voxel;
+    const int32_t a[3] = {sample_voxel(r.px, r.ux, ta, v.inv_voxel),
+                          sample_voxel(r.py, r.uy, ta, v.inv_voxel),
+                          sample_voxel(r.pz, r.uz, ta, v.inv_voxel)};
+    int32_t e[3] = {sample_voxel(r.px, r.ux, te, v.inv_voxel),
+                    sample_voxel(r.py, r.uy, te, v.inv_voxel),
+                    sample_voxel(r.pz, r.uz, te, v.inv_voxel)};
+    const bool cull_last = sample_b(r, S, te) >= r.thr && sample_culled(v, src, r, e[0], e[1], e[2]);
+    int b = S;
+    if (cull_last) {  // the range ends at S - 1
+        const float t = (float)(S - 1) * v.voxel;
+        e[0] = sample_voxel(r.px, r.ux, t, v.inv_voxel);
+        e[1] = sample_voxel(r.py, r.uy, t, v.inv_voxel);
+        e[2] = sample_voxel(r.pz, r.uz, t, v.inv_voxel);
+        b = S - 1;
     }
-    if (!culled) {
-        const int32_t bx = floor_div8(vx), by = floor_div8(vy), bz = floor_div8(vz);
-        const uint32_t dx = uint32_t(bx - ox), dy = uint32_t(by - oy), dz = uint32_t(bz - oz);
-        if ((dx | dy | dz) < 4u) cells |= 1ull << (dx | (dy << 2) | (dz << 4));
-        else lk.push(pack_coord(bx, by, bz));
+    int32_t ka[3], kb[3];
+    bool simple = !other_b;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        ka[q] = floor_div8(a[q]);
+        kb[q] = floor_div8(e[q]);
+        simple &= (uint32_t)(kb[q] - ka[q] + 1) <= 2u;
     }
+    if (!simple) return false;
+    const float p[3] = {r.px, r.py, r.pz}, u[3] = {r.ux, r.uy, r.uz};
+    constexpr int kHalvings = ceil_log2(2 * S);  // the range -S..b has at most 2S steps
+    int c[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        c[q] = b + 1;  // no change along q
+        if (kb[q] != ka[q]) {
+            int lo = -S, hi = b;  // block ka at lo, kb at hi
+#pragma unroll
+            for (int i = 0; i < kHalvings; ++i) {
+                if (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    const int32_t km = floor_div8(sample_voxel(p[q], u[q], (float)mid * v.voxel,
+                                                               v.inv_voxel));
+                    if (km == kb[q]) hi = mid;
+                    else lo = mid;
+                }
+            }
+            c[q] = hi;
+        }
+    }
+    mark_block(ka[0], ka[1], ka[2], ox, oy, oz, cells, lk);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        bool fresh = c[q] <= b;
+        for (int j = 0; j < q; ++j) fresh &= c[j] != c[q];
+        if (fresh)
+            mark_block(c[q] >= c[0] ? kb[0] : ka[0], c[q] >= c[1] ? kb[1] : ka[1],
+                       c[q] >= c[2] ? kb[2] : ka[2], ox, oy, oz, cells, lk);
+    }
+    return true;
 }
 
 // Walks one point per lane (live lanes only) and hands every distinct block
@@ -424,27 +551,20 @@ __device__ void walk_points(const TileCtx& c, const PointSrc& src, bool ok, floa
     const int32_t oz = __builtin_amdgcn_readlane(floor_div8((int32_t)floorf(pz * v.inv_voxel)), leader) - 1;
     unsigned long long cells = 0;
     LaneKeys lk;
-    if (STEPS > 0) {
-        // A ray's 2 * STEPS + 1 samples span 2 * STEPS <= 8 voxels, so they
-        // meet at most 2 blocks per axis, 4 along the ray: the lane buffer
-        // (kKeysPerLane keys) never fills.
-        static_assert(STEPS <= 4 && kKeysPerLane >= 4, "lane buffer bound");
-        if (ok) {
-#pragma unroll
-            for (int s = -STEPS; s <= STEPS; ++s)
-                mark_sample(v, src, r, s, (float)s * v.voxel, ox, oy, oz, cells, lk);
-        }
-    } else {
+    // The shortcut marks at most 4 blocks per lane (kKeysPerLane); when a
+    // lane cannot take it, the whole wave walks sample by sample, draining
+    // the lane buffers whenever one fills.
+    static_assert(kKeysPerLane >= 4, "lane buffer bound");
+    bool walked = false;
+    if (STEPS > 0 && ok) walked = mark_band<STEPS>(v, src, r, ox, oy, oz, cells, lk);
+    if (STEPS == 0 || __ballot(ok && !walked)) {
         for (int32_t s = -v.steps; s <= v.steps; ++s) {
-            if (ok) mark_sample(v, src, r, s, (float)s * v.voxel, ox, oy, oz, cells, lk);
+            if (ok && !walked) mark_sample(v, src, r, s, (float)s * v.voxel, ox, oy, oz, cells, lk);
             if (__ballot(lk.valid >> (kKeysPerLane - 1))) flush_wave<MODE, TOUCH>(c, lk);
         }
     }
     cells = __ockl_wfred_or_u64(cells);
-    const int lane = __lane_id();
-    process_keys<MODE, TOUCH>(c, (cells >> lane) & 1ull,
-                              pack_coord(ox + (lane & 3), oy + ((lane >> 2) & 3), oz + (lane >> 4)));
-    flush_wave<MODE, TOUCH>(c, lk);
+    process_window_and_buffers<MODE, TOUCH>(c, cells, ox, oy, oz, lk);
 }
 
 template <int MODE, bool TOUCH>

@@ -141,7 +141,7 @@ hipError_t launch_fill_table(HashEntry* table, uint32_t n, hipStream_t s);
 hipError_t launch_cas_ceiling(unsigned long long* slots, uint32_t mask, int32_t n_ops,
                               hipStream_t s);
 
-__host__ __device__ inline int32_t ceil_log2(int32_t x) {
+__host__ __device__ constexpr int32_t ceil_log2(int32_t x) {
     int32_t l = 0;
     while ((int32_t(1) << l) < x) ++l;
     return l;
