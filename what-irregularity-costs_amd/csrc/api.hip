@@ -76,6 +76,7 @@ struct OsnTsdfHipVolume {
     int32_t* ex_vcount = nullptr;
     int32_t* ex_voff = nullptr;
     int32_t* ex_chunks = nullptr;
+    int32_t* ex_nb = nullptr;  // 8 neighbour indices per block
     int32_t* ex_flag_count = nullptr;  // [0] = arrival counter, [1] = vertex count
     int32_t ex_blocks_cap = 0;
     float* stage_posnor = nullptr;
@@ -182,7 +183,7 @@ OsnTsdfHipVolume::~OsnTsdfHipVolume() {
     for (auto e : ev_pool) hipEventDestroy(e);
     void* ptrs[] = {d.table, d.ctr, d.block_coord, d.tsdf, d.weight, d.r, d.g, d.b, d.vox,
                     d.new_keys, d.touched, d.sort_a, d.sort_b, d.assign_done, ex_vcount, ex_voff,
-                    ex_chunks, ex_flag_count, stage_posnor, stage_rgb, d_tiled};
+                    ex_chunks, ex_nb, ex_flag_count, stage_posnor, stage_rgb, d_tiled};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (stream) hipStreamDestroy(stream);
@@ -218,8 +219,9 @@ void OsnTsdfHipVolume::stage_end(int stage, hipEvent_t a) {
 bool OsnTsdfHipVolume::grow_extract(int32_t blocks) {
     if (blocks <= ex_blocks_cap && ex_vcount) return true;
     int32_t want = blocks > 1024 ? blocks : 1024;
-    if (ex_vcount) { hipFree(ex_vcount); hipFree(ex_voff); hipFree(ex_chunks); }
-    ex_vcount = ex_voff = ex_chunks = nullptr;
+    if (ex_vcount) { hipFree(ex_vcount); hipFree(ex_voff); hipFree(ex_chunks); hipFree(ex_nb); }
+    ex_vcount = ex_voff = ex_chunks = ex_nb = nullptr;
+    HIP_OK(hipMalloc(&ex_nb, (size_t)want * 8 * sizeof(int32_t)));
     HIP_OK(hipMalloc(&ex_vcount, (size_t)want * sizeof(int32_t)));
     HIP_OK(hipMalloc(&ex_voff, (size_t)want * sizeof(int32_t)));
     HIP_OK(hipMalloc(&ex_chunks, ((size_t)want / 1024 + 2) * sizeof(int32_t)));
@@ -627,6 +629,7 @@ int32_t osn_tsdf_hip_extract_mesh_device(OsnTsdfHipVolume* v, float* d_posnor, u
     a.n_blocks = -1;  // read from the device: no host synchronisation
     a.block_vcount = v->ex_vcount;
     a.block_voff = v->ex_voff;
+    a.block_nb = v->ex_nb;
     a.chunk_sums = v->ex_chunks;
     a.scan_flag = v->ex_flag_count;
     a.posnor = d_posnor;
@@ -655,6 +658,7 @@ int32_t osn_tsdf_hip_extract_mesh(OsnTsdfHipVolume* v, float* buffer_posnor, uin
     a.n_blocks = n_blocks;
     a.block_vcount = v->ex_vcount;
     a.block_voff = v->ex_voff;
+    a.block_nb = v->ex_nb;
     a.chunk_sums = v->ex_chunks;
     a.scan_flag = v->ex_flag_count;
     a.cap = vert_cap;
@@ -1151,6 +1155,7 @@ int32_t osn_tsdf_hip_batch_set_extract(OsnTsdfHipBatch* b, float* const* d_posno
         a.n_blocks = -1;
         a.block_vcount = v->ex_vcount;
         a.block_voff = v->ex_voff;
+        a.block_nb = v->ex_nb;
         a.chunk_sums = v->ex_chunks;
         a.scan_flag = v->ex_flag_count;
         a.posnor = d_posnor[s];

@@ -9,8 +9,10 @@
 // the 0-7 diagonal, position-ordered interpolation, one face normal per
 // triangle. The reference launches one thread per voxel and performs 8 hash
 // lookups per voxel (cpp/src/marching_tets_cuda.cu:73-167); here a workgroup
-// performs the 7 neighbour-block lookups once, stages the block's 9x9x9
-// corner samples in LDS and classifies from there.
+// stages the block's 9x9x9 corner samples in LDS and classifies from there.
+// The count pass looks the 7 neighbour blocks up once per block and keeps
+// their indices for the emit pass, which assembles each round's vertices in
+// LDS and writes them out as one contiguous, coalesced run.
 //
 // Compiled with -ffp-contract=off so vertex arithmetic matches the reference
 // CPU extractor's rounding.
@@ -26,28 +28,44 @@ constexpr int kExThreads = 256;
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanChunk = 1024;
 
-__constant__ int c_tets[6][4] = {
-    {0, 5, 1, 7}, {0, 1, 3, 7}, {0, 3, 2, 7}, {0, 6, 4, 7}, {0, 4, 5, 7}, {0, 2, 6, 7},
-};
 
+constexpr int kStageFloats = 4096;  // emit staging: 682 vertices of 6 floats (16 KiB)
+
+// The 9x9x9 corner samples of one block; colour planes only where the
+// storage keeps colour (the reference's weighted sums).
+template <int STORE>
 struct Cube {
     float val[kCube];     // mean tsdf, NaN when the corner is not a valid sample
-    float col[3][kCube];  // mean colour (sums mode only)
     int32_t nb[8];        // pool index of the block at offset (x, y, z) bits
+};
+template <>
+struct Cube<kStoreSums> {
+    float val[kCube];
+    int32_t nb[8];
+    float col[3][kCube];  // mean colour
 };
 
 __device__ inline int cube_index(int x, int y, int z) { return (z * 9 + y) * 9 + x; }
 
-template <int STORE>
-__device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube& c) {
+// Stage block bi's corner samples. The count pass (LOOKUP) finds the 7
+// neighbour blocks in the table and records their indices in nb_out[bi * 8];
+// the emit pass reads them back from there.
+template <int STORE, bool LOOKUP>
+__device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>& c,
+                          int32_t* nb_out) {
     const int tid = threadIdx.x;
     if (tid < 8) {
         int32_t nbi = bi;
-        if (tid != 0) {
-            const int32_t bx = v.block_coord[bi * 3 + 0] + (tid & 1);
-            const int32_t by = v.block_coord[bi * 3 + 1] + ((tid >> 1) & 1);
-            const int32_t bz = v.block_coord[bi * 3 + 2] + ((tid >> 2) & 1);
-            nbi = find_index(v, pack_coord(bx, by, bz));
+        if (LOOKUP) {
+            if (tid != 0) {
+                const int32_t bx = v.block_coord[bi * 3 + 0] + (tid & 1);
+                const int32_t by = v.block_coord[bi * 3 + 1] + ((tid >> 1) & 1);
+                const int32_t bz = v.block_coord[bi * 3 + 2] + ((tid >> 2) & 1);
+                nbi = find_index(v, pack_coord(bx, by, bz));
+            }
+            nb_out[(size_t)bi * 8 + tid] = nbi;
+        } else {
+            nbi = nb_out[(size_t)bi * 8 + tid];
         }
         c.nb[tid] = nbi;
     }
@@ -86,55 +104,83 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube& c) {
             }
         }
         c.val[i] = t;
-        c.col[0][i] = cr; c.col[1][i] = cg; c.col[2][i] = cb;
+        if constexpr (STORE == kStoreSums) {
+            c.col[0][i] = cr; c.col[1][i] = cg; c.col[2][i] = cb;
+        }
     }
     __syncthreads();
 }
 
-// Vertex count of voxel (lx, ly, lz)'s cube; fills the corner values.
-__device__ inline int cube_count(const Cube& c, int lx, int ly, int lz, float iso, float val[8]) {
+// The 6 tetrahedra (corner ids, 4 bits each, lowest first):
+// {0,5,1,7} {0,1,3,7} {0,3,2,7} {0,6,4,7} {0,4,5,7} {0,2,6,7}.
+// Kept as immediates: an array indexed by a lane-varying t would live in
+// scratch memory.
+__device__ inline uint32_t tet_corners(int t) {
+    return t < 4 ? (uint32_t)(0x7460723073107150ull >> (16 * t)) & 0xffffu
+                 : (0x76207540u >> (16 * (t - 4))) & 0xffffu;
+}
+
+// Crossing code of tetrahedron t from the cube's corner mask (bit k set when
+// corner k is below iso), and the vertices it emits: none, one triangle, or
+// two from a quad.
+__device__ inline int tet_code(uint32_t mask, int t) {
+    const uint32_t tc = tet_corners(t);
+    return (int)(((mask >> (tc & 15u)) & 1u) | (((mask >> ((tc >> 4) & 15u)) & 1u) << 1) |
+                 (((mask >> ((tc >> 8) & 15u)) & 1u) << 2) | (((mask >> (tc >> 12)) & 1u) << 3));
+}
+
+__device__ inline int tet_vertices(int code) {
+    return (code == 0 || code == 15) ? 0 : ((code == 3 || code == 5 || code == 6 ||
+                                            code == 9 || code == 10 || code == 12) ? 6 : 3);
+}
+
+// Vertex count of voxel (lx, ly, lz)'s cube (0 when a corner is invalid) and
+// its corner mask.
+template <int STORE>
+__device__ inline int cube_count(const Cube<STORE>& c, int lx, int ly, int lz, float iso,
+                                 uint32_t& mask) {
+    mask = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        val[k] = c.val[cube_index(lx + (k & 1), ly + ((k >> 1) & 1), lz + ((k >> 2) & 1))];
-        if (__builtin_isnan(val[k])) return 0;
+        const float x = c.val[cube_index(lx + (k & 1), ly + ((k >> 1) & 1), lz + ((k >> 2) & 1))];
+        if (__builtin_isnan(x)) return 0;
+        mask |= (x < iso ? 1u : 0u) << k;
     }
     int n = 0;
 #pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        const int code = (val[c_tets[t][0]] < iso ? 1 : 0) | (val[c_tets[t][1]] < iso ? 2 : 0) |
-                         (val[c_tets[t][2]] < iso ? 4 : 0) | (val[c_tets[t][3]] < iso ? 8 : 0);
-        n += (code == 0 || code == 15) ? 0 : ((code == 3 || code == 5 || code == 6 ||
-                                              code == 9 || code == 10 || code == 12) ? 6 : 3);
-    }
+    for (int t = 0; t < 6; ++t) n += tet_vertices(tet_code(mask, t));
     return n;
 }
 
+struct P3 { float x, y, z; };
 struct V6 { float p[3]; float c[3]; };
 
-__device__ inline V6 interp(float iso, const float* pa, const float* pb, float va, float vb,
-                            const float* ca, const float* cb) {
-    const bool sw = (pb[0] < pa[0]) || (pb[0] == pa[0] && pb[1] < pa[1]) ||
-                    (pb[0] == pa[0] && pb[1] == pa[1] && pb[2] < pa[2]);
-    const float* p0 = sw ? pb : pa;
-    const float* p1 = sw ? pa : pb;
-    const float* c0 = sw ? cb : ca;
-    const float* c1 = sw ? ca : cb;
+template <bool COL>
+__device__ inline V6 interp(float iso, P3 pa, P3 pb, float va, float vb, P3 ca, P3 cb) {
+    const bool sw = (pb.x < pa.x) || (pb.x == pa.x && pb.y < pa.y) ||
+                    (pb.x == pa.x && pb.y == pa.y && pb.z < pa.z);
+    const P3 p0 = sw ? pb : pa, p1 = sw ? pa : pb;
+    const P3 c0 = sw ? cb : ca, c1 = sw ? ca : cb;
     const float v0 = sw ? vb : va;
     const float v1 = sw ? va : vb;
     const float den = v1 - v0;
     float t = fabsf(den) < 1e-12f ? 0.5f : (iso - v0) / den;
     t = fminf(fmaxf(t, 0.0f), 1.0f);
     V6 o;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        o.p[k] = p0[k] + t * (p1[k] - p0[k]);
-        o.c[k] = c0[k] + t * (c1[k] - c0[k]);
-    }
+    o.p[0] = p0.x + t * (p1.x - p0.x);
+    o.p[1] = p0.y + t * (p1.y - p0.y);
+    o.p[2] = p0.z + t * (p1.z - p0.z);
+    o.c[0] = COL ? c0.x + t * (c1.x - c0.x) : 128.0f;
+    o.c[1] = COL ? c0.y + t * (c1.y - c0.y) : 128.0f;
+    o.c[2] = COL ? c0.z + t * (c1.z - c0.z) : 128.0f;
     return o;
 }
 
+// Triangle (a, b, c) with its face normal: positions and normals to posnor
+// from float index at_pn * 6 (global memory or the LDS staging area), colours
+// to rgb from byte index at_rgb * 3 when rgb is set.
 __device__ inline void write_tri(const V6& a, const V6& b, const V6& c, float* posnor,
-                                 uint8_t* rgb, int64_t at) {
+                                 int64_t at_pn, uint8_t* rgb, int64_t at_rgb) {
     const float e1[3] = {b.p[0] - a.p[0], b.p[1] - a.p[1], b.p[2] - a.p[2]};
     const float e2[3] = {c.p[0] - a.p[0], c.p[1] - a.p[1], c.p[2] - a.p[2]};
     float nx = e1[1] * e2[2] - e1[2] * e2[1];
@@ -145,11 +191,11 @@ __device__ inline void write_tri(const V6& a, const V6& b, const V6& c, float* p
     const V6* vs[3] = {&a, &b, &c};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float* d = posnor + (size_t)(at + k) * 6;
+        float* d = posnor + (size_t)(at_pn + k) * 6;
         d[0] = vs[k]->p[0]; d[1] = vs[k]->p[1]; d[2] = vs[k]->p[2];
         d[3] = nx; d[4] = ny; d[5] = nz;
         if (rgb) {
-            uint8_t* q = rgb + (size_t)(at + k) * 3;
+            uint8_t* q = rgb + (size_t)(at_rgb + k) * 3;
             q[0] = (uint8_t)fminf(fmaxf(vs[k]->c[0], 0.0f), 255.0f);
             q[1] = (uint8_t)fminf(fmaxf(vs[k]->c[1], 0.0f), 255.0f);
             q[2] = (uint8_t)fminf(fmaxf(vs[k]->c[2], 0.0f), 255.0f);
@@ -157,54 +203,75 @@ __device__ inline void write_tri(const V6& a, const V6& b, const V6& c, float* p
     }
 }
 
-__device__ void emit_cube(const Cube& c, const VolDev& v, int32_t bx, int32_t by, int32_t bz,
-                          int lx, int ly, int lz, float iso, const float val[8], float* posnor,
-                          uint8_t* rgb, int64_t at) {
-    float pos[8][3], col[8][3];
+// The triangles of tetrahedron t of voxel (lx, ly, lz)'s cube (corner mask
+// `mask`), written from vertex index at (posnor) and at_rgb (rgb). Corner
+// positions, values and colours are formed per corner id, never held in an
+// array indexed by a lane-varying value.
+template <int STORE>
+__device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int32_t by,
+                         int32_t bz, int lx, int ly, int lz, float iso, uint32_t mask, int t,
+                         float* posnor, int64_t at, uint8_t* rgb, int64_t at_rgb) {
+    constexpr bool kCol = STORE == kStoreSums;
     const int32_t vx = bx * 8 + lx, vy = by * 8 + ly, vz = bz * 8 + lz;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-        pos[k][0] = ((float)(vx + ox) + 0.5f) * v.voxel;
-        pos[k][1] = ((float)(vy + oy) + 0.5f) * v.voxel;
-        pos[k][2] = ((float)(vz + oz) + 0.5f) * v.voxel;
-        const int ci = cube_index(lx + ox, ly + oy, lz + oz);
-        col[k][0] = c.col[0][ci]; col[k][1] = c.col[1][ci]; col[k][2] = c.col[2][ci];
+    auto pos = [&](int k) {
+        return P3{((float)(vx + (k & 1)) + 0.5f) * v.voxel,
+                  ((float)(vy + ((k >> 1) & 1)) + 0.5f) * v.voxel,
+                  ((float)(vz + ((k >> 2) & 1)) + 0.5f) * v.voxel};
+    };
+    auto ci = [&](int k) { return cube_index(lx + (k & 1), ly + ((k >> 1) & 1), lz + ((k >> 2) & 1)); };
+    auto val = [&](int k) { return c.val[ci(k)]; };
+    auto col = [&](int k) {
+        if constexpr (kCol) return P3{c.col[0][ci(k)], c.col[1][ci(k)], c.col[2][ci(k)]};
+        else return P3{128.0f, 128.0f, 128.0f};
+    };
+    const uint32_t tc = tet_corners(t);
+    const int i0 = (int)(tc & 15u), i1 = (int)((tc >> 4) & 15u), i2 = (int)((tc >> 8) & 15u),
+              i3 = (int)(tc >> 12);
+    int a, b, cc, d;
+    bool quad;
+    switch (tet_code(mask, t)) {
+        case 0x01: case 0x0E: a = i0; b = i1; cc = i2; d = i3; quad = false; break;
+        case 0x02: case 0x0D: a = i1; b = i0; cc = i2; d = i3; quad = false; break;
+        case 0x04: case 0x0B: a = i2; b = i0; cc = i1; d = i3; quad = false; break;
+        case 0x08: case 0x07: a = i3; b = i0; cc = i1; d = i2; quad = false; break;
+        case 0x03: a = i0; b = i1; cc = i2; d = i3; quad = true; break;
+        case 0x0C: a = i2; b = i3; cc = i0; d = i1; quad = true; break;
+        case 0x05: a = i0; b = i2; cc = i1; d = i3; quad = true; break;
+        case 0x0A: a = i1; b = i3; cc = i0; d = i2; quad = true; break;
+        case 0x06: a = i1; b = i2; cc = i0; d = i3; quad = true; break;
+        case 0x09: a = i0; b = i3; cc = i1; d = i2; quad = true; break;
+        default: return;
     }
+    const P3 pa = pos(a), pb = pos(b), pc = pos(cc), pd = pos(d);
+    const float va = val(a), vb = val(b), vc = val(cc), vd = val(d);
+    const P3 ca = col(a), cb = col(b), c3 = col(cc), cd = col(d);
+    if (!quad) {
+        const V6 e0 = interp<kCol>(iso, pa, pb, va, vb, ca, cb);
+        const V6 e1 = interp<kCol>(iso, pa, pc, va, vc, ca, c3);
+        const V6 e2 = interp<kCol>(iso, pa, pd, va, vd, ca, cd);
+        write_tri(e0, e1, e2, posnor, at, rgb, at_rgb);
+    } else {
+        const V6 ac = interp<kCol>(iso, pa, pc, va, vc, ca, c3);
+        const V6 ad = interp<kCol>(iso, pa, pd, va, vd, ca, cd);
+        const V6 bd = interp<kCol>(iso, pb, pd, vb, vd, cb, cd);
+        const V6 bc = interp<kCol>(iso, pb, pc, vb, vc, cb, c3);
+        write_tri(ac, ad, bd, posnor, at, rgb, at_rgb);
+        write_tri(ac, bd, bc, posnor, at + 3, rgb, at_rgb + 3);
+    }
+}
+
+// Every tetrahedron of one cube, in order.
+template <int STORE>
+__device__ void emit_cube(const Cube<STORE>& c, const VolDev& v, int32_t bx, int32_t by,
+                          int32_t bz, int lx, int ly, int lz, float iso, uint32_t mask,
+                          float* posnor, int64_t at, uint8_t* rgb, int64_t at_rgb) {
+#pragma unroll
     for (int t = 0; t < 6; ++t) {
-        const int i0 = c_tets[t][0], i1 = c_tets[t][1], i2 = c_tets[t][2], i3 = c_tets[t][3];
-        const int code = (val[i0] < iso ? 1 : 0) | (val[i1] < iso ? 2 : 0) |
-                         (val[i2] < iso ? 4 : 0) | (val[i3] < iso ? 8 : 0);
-        int a, b, cc, d;
-        bool quad;
-        switch (code) {
-            case 0x01: case 0x0E: a = i0; b = i1; cc = i2; d = i3; quad = false; break;
-            case 0x02: case 0x0D: a = i1; b = i0; cc = i2; d = i3; quad = false; break;
-            case 0x04: case 0x0B: a = i2; b = i0; cc = i1; d = i3; quad = false; break;
-            case 0x08: case 0x07: a = i3; b = i0; cc = i1; d = i2; quad = false; break;
-            case 0x03: a = i0; b = i1; cc = i2; d = i3; quad = true; break;
-            case 0x0C: a = i2; b = i3; cc = i0; d = i1; quad = true; break;
-            case 0x05: a = i0; b = i2; cc = i1; d = i3; quad = true; break;
-            case 0x0A: a = i1; b = i3; cc = i0; d = i2; quad = true; break;
-            case 0x06: a = i1; b = i2; cc = i0; d = i3; quad = true; break;
-            case 0x09: a = i0; b = i3; cc = i1; d = i2; quad = true; break;
-            default: continue;
-        }
-        if (!quad) {
-            const V6 e0 = interp(iso, pos[a], pos[b], val[a], val[b], col[a], col[b]);
-            const V6 e1 = interp(iso, pos[a], pos[cc], val[a], val[cc], col[a], col[cc]);
-            const V6 e2 = interp(iso, pos[a], pos[d], val[a], val[d], col[a], col[d]);
-            write_tri(e0, e1, e2, posnor, rgb, at);
-            at += 3;
-        } else {
-            const V6 ac = interp(iso, pos[a], pos[cc], val[a], val[cc], col[a], col[cc]);
-            const V6 ad = interp(iso, pos[a], pos[d], val[a], val[d], col[a], col[d]);
-            const V6 bd = interp(iso, pos[b], pos[d], val[b], val[d], col[b], col[d]);
-            const V6 bc = interp(iso, pos[b], pos[cc], val[b], val[cc], col[b], col[cc]);
-            write_tri(ac, ad, bd, posnor, rgb, at);
-            write_tri(ac, bd, bc, posnor, rgb, at + 3);
-            at += 6;
-        }
+        const int n = tet_vertices(tet_code(mask, t));
+        if (!n) continue;
+        emit_tet(c, v, bx, by, bz, lx, ly, lz, iso, mask, t, posnor, at, rgb, at_rgb);
+        at += n;
+        at_rgb += n;
     }
 }
 
@@ -237,15 +304,15 @@ __device__ inline int wg_exclusive_scan(int x, int* s_wave, int& total) {
 
 template <int STORE>
 __global__ __launch_bounds__(kExThreads) void k_extract_count(VolDev v, ExtractArgs a) {
-    __shared__ Cube c;
+    __shared__ Cube<STORE> c;
     __shared__ int s_wave[kExThreads / 64];
     const int32_t n = n_blocks_of(v, a);
     for (int32_t b = blockIdx.x; b < n; b += gridDim.x) {
-        load_cube<STORE>(v, b, a.min_weight, c);
+        load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb);
         int cnt = 0;
         for (int vi = threadIdx.x; vi < kBlockVoxels; vi += kExThreads) {
-            float val[8];
-            cnt += cube_count(c, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, val);
+            uint32_t mask;
+            cnt += cube_count(c, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, mask);
         }
         int total;
         wg_exclusive_scan(cnt, s_wave, total);
@@ -302,15 +369,22 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan(VolDev v, ExtractAr
     }
 }
 
+// Emit: per block, two rounds of 256 voxels. A round's vertices are one
+// contiguous run of the output (its offset from the scans); the workgroup
+// writes them into LDS, then copies the run out 8 bytes per lane, so the
+// stores are coalesced. A round too large for the staging area writes its
+// vertices straight to the output.
 template <int STORE>
 __global__ __launch_bounds__(kExThreads) void k_extract_emit(VolDev v, ExtractArgs a) {
-    __shared__ Cube c;
+    __shared__ Cube<STORE> c;
     __shared__ int s_wave[kExThreads / 64];
+    __shared__ float2 s_stage[kStageFloats / 2];
+    __shared__ uint32_t s_tets[kExThreads * 6];
     if (*a.count_out < 0) return;  // overflow: write nothing
     const int32_t n = n_blocks_of(v, a);
     for (int32_t b = blockIdx.x; b < n; b += gridDim.x) {
         if (a.block_vcount[b] == 0) continue;
-        load_cube<STORE>(v, b, a.min_weight, c);
+        load_cube<STORE, false>(v, b, a.min_weight, c, a.block_nb);
         const int32_t bx = v.block_coord[b * 3 + 0];
         const int32_t by = v.block_coord[b * 3 + 1];
         const int32_t bz = v.block_coord[b * 3 + 2];
@@ -318,11 +392,62 @@ __global__ __launch_bounds__(kExThreads) void k_extract_emit(VolDev v, ExtractAr
         for (int r = 0; r < kBlockVoxels / kExThreads; ++r) {
             const int vi = r * kExThreads + threadIdx.x;
             const int lx = vi & 7, ly = (vi >> 3) & 7, lz = vi >> 6;
-            float val[8];
-            const int cnt = cube_count(c, lx, ly, lz, a.iso, val);
-            int total;
-            const int off = wg_exclusive_scan(cnt, s_wave, total);
-            if (cnt) emit_cube(c, v, bx, by, bz, lx, ly, lz, a.iso, val, a.posnor, a.rgb, base + off);
+            uint32_t mask;
+            const int cnt = cube_count(c, lx, ly, lz, a.iso, mask);
+            int nt = 0;         // tetrahedra with a crossing
+            uint32_t tv = 0;    // 2 bits per tetrahedron: its vertex count / 3
+            if (cnt) {
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    const int nv = tet_vertices(tet_code(mask, t));
+                    tv |= (uint32_t)(nv / 3) << (2 * t);
+                    nt += nv != 0;
+                }
+            }
+            // one scan of both counts: vertices (low 16 bits, <= 256 x 36)
+            // and tetrahedra (high bits, <= 256 x 6)
+            int packed_total;
+            const int packed_off = wg_exclusive_scan(cnt | (nt << 16), s_wave, packed_total);
+            const int off = packed_off & 0xffff, total = packed_total & 0xffff;
+            const int toff = packed_off >> 16, n_tets = packed_total >> 16;
+            if (total * 6 <= kStageFloats) {
+                // List the round's crossing tetrahedra with their vertex
+                // offsets, then emit one tetrahedron per lane: a lane per
+                // voxel would leave most lanes of a wave idle while a few
+                // emit many triangles.
+                int vo = off, k = toff;
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    const int nv = (int)((tv >> (2 * t)) & 3u) * 3;
+                    if (nv) {
+                        s_tets[k++] = ((uint32_t)vo << 12) | ((uint32_t)t << 9) | (uint32_t)vi;
+                        vo += nv;
+                    }
+                }
+                __syncthreads();
+                for (int i = threadIdx.x; i < n_tets; i += kExThreads) {
+                    const uint32_t it = s_tets[i];
+                    const int vj = (int)(it & 511u), t = (int)((it >> 9) & 7u), at = (int)(it >> 12);
+                    const int jx = vj & 7, jy = (vj >> 3) & 7, jz = vj >> 6;
+                    uint32_t jm;
+                    cube_count(c, jx, jy, jz, a.iso, jm);
+                    emit_tet(c, v, bx, by, bz, jx, jy, jz, a.iso, jm, t,
+                             reinterpret_cast<float*>(s_stage), at, a.rgb, base + at);
+                }
+                __syncthreads();
+                float* out = a.posnor + base * 6;
+                if ((reinterpret_cast<uintptr_t>(out) & 7u) == 0) {
+                    float2* dst = reinterpret_cast<float2*>(out);
+                    for (int i = threadIdx.x; i < total * 3; i += kExThreads) dst[i] = s_stage[i];
+                } else {  // a caller's buffer need only be float-aligned
+                    const float* src = reinterpret_cast<const float*>(s_stage);
+                    for (int i = threadIdx.x; i < total * 6; i += kExThreads) out[i] = src[i];
+                }
+                __syncthreads();
+            } else if (cnt) {
+                emit_cube(c, v, bx, by, bz, lx, ly, lz, a.iso, mask, a.posnor, base + off, a.rgb,
+                          base + off);
+            }
             base += total;
         }
         __syncthreads();

@@ -89,6 +89,7 @@ struct ExtractArgs {
     int32_t n_blocks;      // blocks to walk (<= block_count), -1 = read block_count
     int32_t* block_vcount; // per block vertex counts
     int32_t* block_voff;   // per block exclusive offsets (emit reads)
+    int32_t* block_nb;     // per block the pool indices of its 8 neighbours (count writes)
     int32_t* chunk_sums;   // per 1024-block chunk sums
     int32_t* scan_flag;    // arrival counter for the chunk scan
     float* posnor;         // device output, 6 floats per vertex
