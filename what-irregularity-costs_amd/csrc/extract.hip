@@ -24,7 +24,9 @@
 namespace osn_hip {
 namespace {
 
-constexpr int kExThreads = 256;
+constexpr int kExThreads = 256;     // emit workgroup
+constexpr int kCountThreads = 64;   // count workgroup: one wave per block, many blocks per CU
+constexpr int kEmitWavesPerSimd = 6;  // emit's register bound (measured faster than 4 at 99 VGPRs)
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanChunk = 1024;
 
@@ -70,7 +72,7 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
         c.nb[tid] = nbi;
     }
     __syncthreads();
-    for (int i = tid; i < kCube; i += kExThreads) {
+    for (int i = tid; i < kCube; i += blockDim.x) {
         const int x = i % 9, y = (i / 9) % 9, z = i / 81;
         const int o = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
         const int32_t nbi = c.nb[o];
@@ -303,14 +305,14 @@ __device__ inline int wg_exclusive_scan(int x, int* s_wave, int& total) {
 }
 
 template <int STORE>
-__global__ __launch_bounds__(kExThreads) void k_extract_count(VolDev v, ExtractArgs a) {
+__global__ __launch_bounds__(kCountThreads) void k_extract_count(VolDev v, ExtractArgs a) {
     __shared__ Cube<STORE> c;
-    __shared__ int s_wave[kExThreads / 64];
+    __shared__ int s_wave[kCountThreads / 64];
     const int32_t n = n_blocks_of(v, a);
     for (int32_t b = blockIdx.x; b < n; b += gridDim.x) {
         load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb);
         int cnt = 0;
-        for (int vi = threadIdx.x; vi < kBlockVoxels; vi += kExThreads) {
+        for (int vi = threadIdx.x; vi < kBlockVoxels; vi += kCountThreads) {
             uint32_t mask;
             cnt += cube_count(c, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, mask);
         }
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan(VolDev v, ExtractAr
 // stores are coalesced. A round too large for the staging area writes its
 // vertices straight to the output.
 template <int STORE>
-__global__ __launch_bounds__(kExThreads) void k_extract_emit(VolDev v, ExtractArgs a) {
+__global__ __launch_bounds__(kExThreads, kEmitWavesPerSimd) void k_extract_emit(VolDev v, ExtractArgs a) {
     __shared__ Cube<STORE> c;
     __shared__ int s_wave[kExThreads / 64];
     __shared__ float2 s_stage[kStageFloats / 2];
@@ -461,15 +463,19 @@ hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipS
     // otherwise a grid-stride over the pool sized for the whole chip
     const int32_t grid_blocks = a.n_blocks >= 0 ? (a.n_blocks > 0 ? a.n_blocks : 1)
                                                 : (v.pool_capacity < 8192 ? v.pool_capacity : 8192);
+    // the one-wave count workgroups: four times the chip's worth when the
+    // block count is read on the device
+    const int32_t count_blocks = a.n_blocks >= 0 ? grid_blocks
+                                                 : (v.pool_capacity < 32768 ? v.pool_capacity : 32768);
     const int32_t n_for_scan = a.n_blocks >= 0 ? a.n_blocks : v.pool_capacity;
     const int32_t scan_grid = (n_for_scan + kScanChunk - 1) / kScanChunk > 0
                                   ? (n_for_scan + kScanChunk - 1) / kScanChunk : 1;
     if (!emit) {
         switch (v.storage) {
-            case kStoreSums: k_extract_count<kStoreSums><<<grid_blocks, kExThreads, 0, s>>>(v, a); break;
-            case kStoreF32: k_extract_count<kStoreF32><<<grid_blocks, kExThreads, 0, s>>>(v, a); break;
-            case kStoreF16: k_extract_count<kStoreF16><<<grid_blocks, kExThreads, 0, s>>>(v, a); break;
-            case kStoreF64: k_extract_count<kStoreF64><<<grid_blocks, kExThreads, 0, s>>>(v, a); break;
+            case kStoreSums: k_extract_count<kStoreSums><<<count_blocks, kCountThreads, 0, s>>>(v, a); break;
+            case kStoreF32: k_extract_count<kStoreF32><<<count_blocks, kCountThreads, 0, s>>>(v, a); break;
+            case kStoreF16: k_extract_count<kStoreF16><<<count_blocks, kCountThreads, 0, s>>>(v, a); break;
+            case kStoreF64: k_extract_count<kStoreF64><<<count_blocks, kCountThreads, 0, s>>>(v, a); break;
             default: return hipErrorInvalidValue;
         }
         k_extract_scan<<<scan_grid, kScanChunk, 0, s>>>(v, a);
