@@ -9,10 +9,10 @@
 //   rmw-next  the same, with the next block's load issued before this block's
 //             store (the update's pipeline)
 //
-// For each: contiguous blocks 0..n-1, and n blocks drawn at random from a
-// pool (sorted, as a touched list in pool order would be). Times are the
-// average of 20 launches from HIP events; GB/s counts the bytes the pattern
-// must move (n * 2 KiB per direction).
+// For each: contiguous blocks 0..n-1, n blocks drawn at random from a pool
+// in ascending order (a list in pool order), and the same blocks in random
+// order. Times are the average of 20 launches from HIP events; GB/s counts
+// the bytes the pattern must move (n * 2 KiB per direction).
 //
 //   rmw_ceiling [n_blocks=100000] [pool_blocks=1048576] [grid=16384]
 #include <hip/hip_runtime.h>
@@ -132,13 +132,14 @@ int main(int argc, char** argv) {
     const double mb = n * 2048.0 / 1e6;
     std::printf("n_blocks %d, pool %d blocks, grid %d x %d threads\n", n, pool_blocks, grid,
                 kThreads);
-    for (int layout = 0; layout < 2; ++layout) {
+    const char* names[3] = {"contiguous", "random", "unordered"};
+    for (int layout = 0; layout < 3; ++layout) {
         std::vector<int> h(n);
         if (layout == 0)
             std::iota(h.begin(), h.end(), 0);
         else {
             std::copy(ids.begin(), ids.begin() + n, h.begin());
-            std::sort(h.begin(), h.end());
+            if (layout == 1) std::sort(h.begin(), h.end());
         }
         if (!ok(hipMemcpy(list, h.data(), sizeof(int) * n, hipMemcpyHostToDevice), "list copy"))
             return 2;
@@ -149,7 +150,7 @@ int main(int argc, char** argv) {
         if (r < 0 || w < 0 || m < 0 || p < 0) return 3;
         std::printf("%-10s read %7.1f us %5.0f GB/s | write %7.1f us %5.0f GB/s | "
                     "rmw %7.1f us %5.0f GB/s | rmw-next %7.1f us %5.0f GB/s\n",
-                    layout ? "random" : "contiguous", r, mb / r * 1e3, w, mb / w * 1e3, m,
+                    names[layout], r, mb / r * 1e3, w, mb / w * 1e3, m,
                     2 * mb / m * 1e3, p, 2 * mb / p * 1e3);
     }
     hipFree(pool);
