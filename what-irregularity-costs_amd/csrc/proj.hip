@@ -181,14 +181,16 @@ __device__ inline Records<STORE> load_records(const Params& k, const BlockRef& b
     const size_t v0 = (size_t)(b.bi < 0 ? 0 : b.bi) * kBlockVoxels + lane * 4;
     if constexpr (STORE == kStoreF16) {
         const u32x4 OSN_GLOBAL* rec = reinterpret_cast<const u32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 4);
-        R.h[0] = rec[0];
-        R.h[1] = rec[64];  // run 1 starts 256 voxels (1 KiB) later
+        // non-temporal: a record is read once per frame, and the caches are
+        // better left to the depth images the whole chip gathers from
+        R.h[0] = __builtin_nontemporal_load(&rec[0]);
+        R.h[1] = __builtin_nontemporal_load(&rec[64]);  // run 1 starts 256 voxels (1 KiB) later
     } else if constexpr (STORE == kStoreF32) {
         const f32x4 OSN_GLOBAL* rec = reinterpret_cast<const f32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 8);
-        R.h[0][0] = rec[0];
-        R.h[0][1] = rec[1];
-        R.h[1][0] = rec[128];
-        R.h[1][1] = rec[129];
+        R.h[0][0] = __builtin_nontemporal_load(&rec[0]);
+        R.h[0][1] = __builtin_nontemporal_load(&rec[1]);
+        R.h[1][0] = __builtin_nontemporal_load(&rec[128]);
+        R.h[1][1] = __builtin_nontemporal_load(&rec[129]);
     }
     return R;
 }
