@@ -24,14 +24,15 @@
 namespace osn_hip {
 namespace {
 
-constexpr int kExThreads = 256;     // emit workgroup
-constexpr int kCountThreads = 64;   // count workgroup: one wave per block, many blocks per CU
-constexpr int kEmitWavesPerSimd = 6;  // emit's register bound (measured faster than 4 at 99 VGPRs)
+constexpr int kWaveThreads = 64;     // count and emit: one wave per block, many per CU
+// emit's register bound: what its LDS allows per SIMD (4-5 waves; 2 with
+// the colour planes of the weighted sums)
+template <int STORE>
+constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : 5; }
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanChunk = 1024;
-
-
-constexpr int kStageFloats = 4096;  // emit staging: 682 vertices of 6 floats (16 KiB)
+constexpr int kTetCap = 256;          // tetrahedra listed per pass over a block
+constexpr int kStageVerts = 160;      // vertices assembled in LDS per output group
 
 // The 9x9x9 corner samples of one block; colour planes only where the
 // storage keeps colour (the reference's weighted sums).
@@ -122,36 +123,75 @@ __device__ inline uint32_t tet_corners(int t) {
                  : (0x76207540u >> (16 * (t - 4))) & 0xffffu;
 }
 
-// Crossing code of tetrahedron t from the cube's corner mask (bit k set when
-// corner k is below iso), and the vertices it emits: none, one triangle, or
-// two from a quad.
-__device__ inline int tet_code(uint32_t mask, int t) {
-    const uint32_t tc = tet_corners(t);
-    return (int)(((mask >> (tc & 15u)) & 1u) | (((mask >> ((tc >> 4) & 15u)) & 1u) << 1) |
-                 (((mask >> ((tc >> 8) & 15u)) & 1u) << 2) | (((mask >> (tc >> 12)) & 1u) << 3));
-}
+// The same tetrahedra as corner bit sets, one byte each, lowest first.
+constexpr unsigned long long kTetBits = 0xC5B1D18D8BA3ull;
 
-__device__ inline int tet_vertices(int code) {
-    return (code == 0 || code == 15) ? 0 : ((code == 3 || code == 5 || code == 6 ||
-                                            code == 9 || code == 10 || code == 12) ? 6 : 3);
-}
-
-// Vertex count of voxel (lx, ly, lz)'s cube (0 when a corner is invalid) and
-// its corner mask.
-template <int STORE>
-__device__ inline int cube_count(const Cube<STORE>& c, int lx, int ly, int lz, float iso,
-                                 uint32_t& mask) {
-    mask = 0;
+// Classification of a cube from its corner mask (bit k set when corner k is
+// below iso): per tetrahedron t its vertices / 3 in bits 2t..2t+1 (0 none,
+// 1 one triangle, 2 a quad's two), the tetrahedra with a crossing in bits
+// 12-14 and the cube's vertices / 3 in bits 16-19. A tetrahedron with 1 or 3
+// corners below iso cuts one triangle, with 2 a quad.
+__device__ inline uint32_t classify_mask(uint32_t mask) {
+    uint32_t e = 0, nt = 0, n3 = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const float x = c.val[cube_index(lx + (k & 1), ly + ((k >> 1) & 1), lz + ((k >> 2) & 1))];
-        if (__builtin_isnan(x)) return 0;
-        mask |= (x < iso ? 1u : 0u) << k;
+    for (int t = 0; t < 6; ++t) {
+        const int pc = __popc(mask & (uint32_t)((kTetBits >> (8 * t)) & 0xffu));
+        const uint32_t f = pc == 2 ? 2u : (uint32_t)(pc & 1);
+        e |= f << (2 * t);
+        nt += f != 0u;
+        n3 += f;
     }
-    int n = 0;
+    return e | (nt << 12) | (n3 << 16);
+}
+
+// The classification of all 256 corner masks, in LDS; the workgroup fills it
+// once and reads it for every block it walks.
+__device__ inline void build_table(uint32_t* table) {
+    for (int m = threadIdx.x; m < 256; m += blockDim.x) table[m] = classify_mask((uint32_t)m);
+    __syncthreads();
+}
+
+// The row of cubes (0..7, ly, lz) of the staged block: per cube its
+// classification in e[x] (0 when a corner is not a valid sample). Corner
+// columns x and x + 1 are shared by neighbouring cubes, so the row reads 36
+// corners, not 64. Returns the row's tetrahedra with a crossing << 16 |
+// vertices / 3.
+template <int STORE>
+__device__ inline uint32_t classify_row(const Cube<STORE>& c, const uint32_t* table, int ly,
+                                        int lz, float iso, uint32_t (&e)[8]) {
+    uint32_t prev_bits = 0, sum = 0;
+    bool prev_ok = false;
 #pragma unroll
-    for (int t = 0; t < 6; ++t) n += tet_vertices(tet_code(mask, t));
-    return n;
+    for (int x = 0; x < 9; ++x) {
+        // corners (x, ly + dy, lz + dz) at mask bit 2 dy + 4 dz: the cube's
+        // corner ids with dx = 0
+        const float p = c.val[cube_index(x, ly, lz)];
+        const float q = c.val[cube_index(x, ly + 1, lz)];
+        const float r = c.val[cube_index(x, ly, lz + 1)];
+        const float s = c.val[cube_index(x, ly + 1, lz + 1)];
+        const bool ok = !(__builtin_isnan(p) || __builtin_isnan(q) || __builtin_isnan(r) ||
+                          __builtin_isnan(s));
+        const uint32_t bits = (p < iso ? 1u : 0u) | (q < iso ? 4u : 0u) | (r < iso ? 16u : 0u) |
+                              (s < iso ? 64u : 0u);
+        if (x > 0) {
+            const uint32_t ent = (prev_ok && ok) ? table[prev_bits | (bits << 1)] : 0u;
+            e[x - 1] = ent;
+            sum += (ent >> 16) | (((ent >> 12) & 7u) << 16);
+        }
+        prev_bits = bits;
+        prev_ok = ok;
+    }
+    return sum;
+}
+
+__device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x;
 }
 
 struct P3 { float x, y, z; };
@@ -179,8 +219,8 @@ __device__ inline V6 interp(float iso, P3 pa, P3 pb, float va, float vb, P3 ca, 
 }
 
 // Triangle (a, b, c) with its face normal: positions and normals to posnor
-// from float index at_pn * 6 (global memory or the LDS staging area), colours
-// to rgb from byte index at_rgb * 3 when rgb is set.
+// from float index at_pn * 6 (the LDS staging area), colours to rgb from byte
+// index at_rgb * 3 when rgb is set.
 __device__ inline void write_tri(const V6& a, const V6& b, const V6& c, float* posnor,
                                  int64_t at_pn, uint8_t* rgb, int64_t at_rgb) {
     const float e1[3] = {b.p[0] - a.p[0], b.p[1] - a.p[1], b.p[2] - a.p[2]};
@@ -205,13 +245,14 @@ __device__ inline void write_tri(const V6& a, const V6& b, const V6& c, float* p
     }
 }
 
-// The triangles of tetrahedron t of voxel (lx, ly, lz)'s cube (corner mask
-// `mask`), written from vertex index at (posnor) and at_rgb (rgb). Corner
-// positions, values and colours are formed per corner id, never held in an
-// array indexed by a lane-varying value.
+// The triangles of tetrahedron t of voxel (lx, ly, lz)'s cube, written from
+// vertex index at (posnor) and at_rgb (rgb). The crossing code comes from the
+// tetrahedron's own 4 corners (bit j set when its j-th corner is below iso).
+// Corner positions, values and colours are formed per corner id, never held
+// in an array indexed by a lane-varying value.
 template <int STORE>
 __device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int32_t by,
-                         int32_t bz, int lx, int ly, int lz, float iso, uint32_t mask, int t,
+                         int32_t bz, int lx, int ly, int lz, float iso, int t,
                          float* posnor, int64_t at, uint8_t* rgb, int64_t at_rgb) {
     constexpr bool kCol = STORE == kStoreSums;
     const int32_t vx = bx * 8 + lx, vy = by * 8 + ly, vz = bz * 8 + lz;
@@ -229,9 +270,11 @@ __device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int3
     const uint32_t tc = tet_corners(t);
     const int i0 = (int)(tc & 15u), i1 = (int)((tc >> 4) & 15u), i2 = (int)((tc >> 8) & 15u),
               i3 = (int)(tc >> 12);
+    const int code = (val(i0) < iso ? 1 : 0) | (val(i1) < iso ? 2 : 0) |
+                     (val(i2) < iso ? 4 : 0) | (val(i3) < iso ? 8 : 0);
     int a, b, cc, d;
     bool quad;
-    switch (tet_code(mask, t)) {
+    switch (code) {
         case 0x01: case 0x0E: a = i0; b = i1; cc = i2; d = i3; quad = false; break;
         case 0x02: case 0x0D: a = i1; b = i0; cc = i2; d = i3; quad = false; break;
         case 0x04: case 0x0B: a = i2; b = i0; cc = i1; d = i3; quad = false; break;
@@ -262,21 +305,6 @@ __device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int3
     }
 }
 
-// Every tetrahedron of one cube, in order.
-template <int STORE>
-__device__ void emit_cube(const Cube<STORE>& c, const VolDev& v, int32_t bx, int32_t by,
-                          int32_t bz, int lx, int ly, int lz, float iso, uint32_t mask,
-                          float* posnor, int64_t at, uint8_t* rgb, int64_t at_rgb) {
-#pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        const int n = tet_vertices(tet_code(mask, t));
-        if (!n) continue;
-        emit_tet(c, v, bx, by, bz, lx, ly, lz, iso, mask, t, posnor, at, rgb, at_rgb);
-        at += n;
-        at_rgb += n;
-    }
-}
-
 __device__ inline int32_t n_blocks_of(const VolDev& v, const ExtractArgs& a) {
     return a.n_blocks >= 0 ? a.n_blocks : v.ctr->block_count;
 }
@@ -304,22 +332,22 @@ __device__ inline int wg_exclusive_scan(int x, int* s_wave, int& total) {
     return before + incl - x;
 }
 
+// Count: one wave per block, lane = row (ly, lz) of 8 cubes.
 template <int STORE>
-__global__ __launch_bounds__(kCountThreads) void k_extract_count(VolDev v, ExtractArgs a) {
+__global__ __launch_bounds__(kWaveThreads) void k_extract_count(VolDev v, ExtractArgs a) {
     __shared__ Cube<STORE> c;
-    __shared__ int s_wave[kCountThreads / 64];
+    __shared__ uint32_t s_table[256];
+    build_table(s_table);
+    const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
     for (int32_t b = blockIdx.x; b < n; b += gridDim.x) {
         load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb);
-        int cnt = 0;
-        for (int vi = threadIdx.x; vi < kBlockVoxels; vi += kCountThreads) {
-            uint32_t mask;
-            cnt += cube_count(c, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, mask);
-        }
-        int total;
-        wg_exclusive_scan(cnt, s_wave, total);
-        if (threadIdx.x == 0) a.block_vcount[b] = total;
-        __syncthreads();
+        uint32_t e[8];
+        uint32_t n3 = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e) & 0xffffu;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) n3 += __shfl_xor(n3, o);
+        if (lane == 0) a.block_vcount[b] = 3 * (int32_t)n3;
+        __syncthreads();  // the cube is read before the next block replaces it
     }
 }
 
@@ -371,18 +399,20 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan(VolDev v, ExtractAr
     }
 }
 
-// Emit: per block, two rounds of 256 voxels. A round's vertices are one
-// contiguous run of the output (its offset from the scans); the workgroup
-// writes them into LDS, then copies the run out 8 bytes per lane, so the
-// stores are coalesced. A round too large for the staging area writes its
-// vertices straight to the output.
+// Emit: one wave per block, lane = row (ly, lz) of 8 cubes. The block's
+// tetrahedra with a crossing are listed in LDS in output order (voxel, then
+// tetrahedron), each with its vertex offset, and emitted one per lane in
+// groups whose vertices fit the staging area; each group's vertices are one
+// contiguous run of the output, copied out 8 bytes per lane.
 template <int STORE>
-__global__ __launch_bounds__(kExThreads, kEmitWavesPerSimd) void k_extract_emit(VolDev v, ExtractArgs a) {
+__global__ __launch_bounds__(kWaveThreads, emit_waves_per_simd<STORE>()) void k_extract_emit(VolDev v, ExtractArgs a) {
     __shared__ Cube<STORE> c;
-    __shared__ int s_wave[kExThreads / 64];
-    __shared__ float2 s_stage[kStageFloats / 2];
-    __shared__ uint32_t s_tets[kExThreads * 6];
+    __shared__ uint32_t s_table[256];
+    __shared__ uint32_t s_tets[kTetCap];
+    __shared__ float2 s_stage[kStageVerts * 3];
     if (*a.count_out < 0) return;  // overflow: write nothing
+    build_table(s_table);
+    const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
     for (int32_t b = blockIdx.x; b < n; b += gridDim.x) {
         if (a.block_vcount[b] == 0) continue;
@@ -390,101 +420,95 @@ __global__ __launch_bounds__(kExThreads, kEmitWavesPerSimd) void k_extract_emit(
         const int32_t bx = v.block_coord[b * 3 + 0];
         const int32_t by = v.block_coord[b * 3 + 1];
         const int32_t bz = v.block_coord[b * 3 + 2];
-        int64_t base = (int64_t)a.chunk_sums[b / kScanChunk] + a.block_voff[b];
-        for (int r = 0; r < kBlockVoxels / kExThreads; ++r) {
-            const int vi = r * kExThreads + threadIdx.x;
-            const int lx = vi & 7, ly = (vi >> 3) & 7, lz = vi >> 6;
-            uint32_t mask;
-            const int cnt = cube_count(c, lx, ly, lz, a.iso, mask);
-            int nt = 0;         // tetrahedra with a crossing
-            uint32_t tv = 0;    // 2 bits per tetrahedron: its vertex count / 3
-            if (cnt) {
+        const int64_t base = (int64_t)a.chunk_sums[b / kScanChunk] + a.block_voff[b];
+        uint32_t e[8];
+        const uint32_t row = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e);
+        const uint32_t incl = wave_inclusive_scan(row);
+        const int n_tets = (int)(__shfl(incl, 63) >> 16);
+        const uint32_t before = incl - row;
+        // passes of up to kTetCap listed tetrahedra (one pass unless the
+        // block's surface is very dense)
+        for (int g0 = 0; g0 < n_tets; g0 += kTetCap) {
+            // entry: vertex offset in the block (bits 0-14, <= 512 x 36),
+            // quad (15), tetrahedron (16-18), voxel (19-27)
+            int k = (int)(before >> 16);
+            uint32_t vo = 3u * (before & 0xffffu);
 #pragma unroll
-                for (int t = 0; t < 6; ++t) {
-                    const int nv = tet_vertices(tet_code(mask, t));
-                    tv |= (uint32_t)(nv / 3) << (2 * t);
-                    nt += nv != 0;
-                }
-            }
-            // one scan of both counts: vertices (low 16 bits, <= 256 x 36)
-            // and tetrahedra (high bits, <= 256 x 6)
-            int packed_total;
-            const int packed_off = wg_exclusive_scan(cnt | (nt << 16), s_wave, packed_total);
-            const int off = packed_off & 0xffff, total = packed_total & 0xffff;
-            const int toff = packed_off >> 16, n_tets = packed_total >> 16;
-            if (total * 6 <= kStageFloats) {
-                // List the round's crossing tetrahedra with their vertex
-                // offsets, then emit one tetrahedron per lane: a lane per
-                // voxel would leave most lanes of a wave idle while a few
-                // emit many triangles.
-                int vo = off, k = toff;
-#pragma unroll
-                for (int t = 0; t < 6; ++t) {
-                    const int nv = (int)((tv >> (2 * t)) & 3u) * 3;
-                    if (nv) {
-                        s_tets[k++] = ((uint32_t)vo << 12) | ((uint32_t)t << 9) | (uint32_t)vi;
-                        vo += nv;
+            for (int x = 0; x < 8; ++x) {
+                const uint32_t ent = e[x];
+                if (ent & 0xfffu) {
+#pragma nounroll
+                    for (int t = 0; t < 6; ++t) {
+                        const uint32_t f = (ent >> (2 * t)) & 3u;
+                        if (f) {
+                            if (k >= g0 && k < g0 + kTetCap)
+                                s_tets[k - g0] = vo | ((f - 1u) << 15) | ((uint32_t)t << 16) |
+                                                 ((uint32_t)(lane * 8 + x) << 19);
+                            ++k;
+                            vo += 3u * f;
+                        }
                     }
                 }
-                __syncthreads();
-                for (int i = threadIdx.x; i < n_tets; i += kExThreads) {
-                    const uint32_t it = s_tets[i];
-                    const int vj = (int)(it & 511u), t = (int)((it >> 9) & 7u), at = (int)(it >> 12);
-                    const int jx = vj & 7, jy = (vj >> 3) & 7, jz = vj >> 6;
-                    uint32_t jm;
-                    cube_count(c, jx, jy, jz, a.iso, jm);
-                    emit_tet(c, v, bx, by, bz, jx, jy, jz, a.iso, jm, t,
-                             reinterpret_cast<float*>(s_stage), at, a.rgb, base + at);
+            }
+            __syncthreads();
+            const int n_pass = min(kTetCap, n_tets - g0);
+            for (int g = 0; g < n_pass;) {
+                const int i = g + lane;
+                const uint32_t ent = i < n_pass ? s_tets[i] : 0u;
+                const int vo = (int)(ent & 0x7fffu);
+                const int v_lo = __shfl(vo, 0);
+                const int rel_end = vo + (((ent >> 15) & 1u) ? 6 : 3) - v_lo;
+                // the group: the longest run of listed tetrahedra whose
+                // vertices fit the staging area (at least one)
+                const unsigned long long fit = __ballot(i < n_pass && rel_end <= kStageVerts);
+                const int cnt = ~fit == 0ull ? 64 : __builtin_ctzll(~fit);
+                if (lane < cnt) {
+                    const int vi = (int)(ent >> 19), t = (int)((ent >> 16) & 7u);
+                    emit_tet(c, v, bx, by, bz, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, t,
+                             reinterpret_cast<float*>(s_stage), vo - v_lo, a.rgb, base + vo);
                 }
+                const int n_verts = __shfl(rel_end, cnt - 1);
                 __syncthreads();
-                float* out = a.posnor + base * 6;
+                float* out = a.posnor + (base + v_lo) * 6;
                 if ((reinterpret_cast<uintptr_t>(out) & 7u) == 0) {
                     float2* dst = reinterpret_cast<float2*>(out);
-                    for (int i = threadIdx.x; i < total * 3; i += kExThreads) dst[i] = s_stage[i];
+                    for (int j = lane; j < n_verts * 3; j += kWaveThreads) dst[j] = s_stage[j];
                 } else {  // a caller's buffer need only be float-aligned
                     const float* src = reinterpret_cast<const float*>(s_stage);
-                    for (int i = threadIdx.x; i < total * 6; i += kExThreads) out[i] = src[i];
+                    for (int j = lane; j < n_verts * 6; j += kWaveThreads) out[j] = src[j];
                 }
                 __syncthreads();
-            } else if (cnt) {
-                emit_cube(c, v, bx, by, bz, lx, ly, lz, a.iso, mask, a.posnor, base + off, a.rgb,
-                          base + off);
+                g += cnt;
             }
-            base += total;
         }
-        __syncthreads();
     }
 }
 
 }  // namespace
 
 hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipStream_t s) {
-    // grid: one workgroup per block when the count is known on the host,
-    // otherwise a grid-stride over the pool sized for the whole chip
+    // one-wave workgroups: one per block when the count is known on the host,
+    // otherwise a grid-stride over the pool, four times the chip's worth
     const int32_t grid_blocks = a.n_blocks >= 0 ? (a.n_blocks > 0 ? a.n_blocks : 1)
-                                                : (v.pool_capacity < 8192 ? v.pool_capacity : 8192);
-    // the one-wave count workgroups: four times the chip's worth when the
-    // block count is read on the device
-    const int32_t count_blocks = a.n_blocks >= 0 ? grid_blocks
-                                                 : (v.pool_capacity < 32768 ? v.pool_capacity : 32768);
+                                                : (v.pool_capacity < 32768 ? v.pool_capacity : 32768);
     const int32_t n_for_scan = a.n_blocks >= 0 ? a.n_blocks : v.pool_capacity;
     const int32_t scan_grid = (n_for_scan + kScanChunk - 1) / kScanChunk > 0
                                   ? (n_for_scan + kScanChunk - 1) / kScanChunk : 1;
     if (!emit) {
         switch (v.storage) {
-            case kStoreSums: k_extract_count<kStoreSums><<<count_blocks, kCountThreads, 0, s>>>(v, a); break;
-            case kStoreF32: k_extract_count<kStoreF32><<<count_blocks, kCountThreads, 0, s>>>(v, a); break;
-            case kStoreF16: k_extract_count<kStoreF16><<<count_blocks, kCountThreads, 0, s>>>(v, a); break;
-            case kStoreF64: k_extract_count<kStoreF64><<<count_blocks, kCountThreads, 0, s>>>(v, a); break;
+            case kStoreSums: k_extract_count<kStoreSums><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
+            case kStoreF32: k_extract_count<kStoreF32><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
+            case kStoreF16: k_extract_count<kStoreF16><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
+            case kStoreF64: k_extract_count<kStoreF64><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
             default: return hipErrorInvalidValue;
         }
         k_extract_scan<<<scan_grid, kScanChunk, 0, s>>>(v, a);
     } else {
         switch (v.storage) {
-            case kStoreSums: k_extract_emit<kStoreSums><<<grid_blocks, kExThreads, 0, s>>>(v, a); break;
-            case kStoreF32: k_extract_emit<kStoreF32><<<grid_blocks, kExThreads, 0, s>>>(v, a); break;
-            case kStoreF16: k_extract_emit<kStoreF16><<<grid_blocks, kExThreads, 0, s>>>(v, a); break;
-            case kStoreF64: k_extract_emit<kStoreF64><<<grid_blocks, kExThreads, 0, s>>>(v, a); break;
+            case kStoreSums: k_extract_emit<kStoreSums><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
+            case kStoreF32: k_extract_emit<kStoreF32><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
+            case kStoreF16: k_extract_emit<kStoreF16><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
+            case kStoreF64: k_extract_emit<kStoreF64><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
             default: return hipErrorInvalidValue;
         }
     }
