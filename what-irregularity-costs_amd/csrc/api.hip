@@ -865,6 +865,7 @@ struct OsnTsdfHipBatch {
     int32_t cursor = 0;
     int32_t total_tiles = 0;
     uint32_t max_capacity = 0;
+    uint32_t max_pool_capacity = 0;
     std::vector<int64_t> base_calls;
     int32_t alloc_grid = 0, update_grid = 0, update_threads = 256, lds_set = 1024;
     hipGraph_t graph = nullptr;
@@ -872,7 +873,9 @@ struct OsnTsdfHipBatch {
     // mesh extraction every `extract_every` steps (0: never), per scene into
     // caller-owned device buffers; steps that extract replay their own graph
     std::vector<ExtractArgs> ex_args;
+    ExtractArgs* d_ex_args = nullptr;  // the same, on the device, for the batched launches
     int32_t extract_every = 0;
+    int32_t extract_grid = kExtractGridPerScene;
     hipGraph_t graph_x = nullptr;
     hipGraphExec_t exec_x = nullptr;
     bool timing = false;
@@ -894,15 +897,16 @@ struct OsnTsdfHipBatch {
         return extract_every > 0 && (step + 1) % extract_every == 0;
     }
     // Count, scan and emit every scene's mesh (device-side block counts, no
-    // host synchronisation).
+    // host synchronisation): one launch per pass for the whole batch.
     hipError_t enqueue_extract() {
         hipEvent_t e[2] = {};
         if (timing && hipEventCreate(&e[0]) == hipSuccess) hipEventRecord(e[0], stream);
-        hipError_t err = hipSuccess;
-        for (size_t s = 0; s < vols.size() && err == hipSuccess; ++s) {
-            err = launch_extract(vols[s]->d, ex_args[s], false, stream);
-            if (err == hipSuccess) err = launch_extract(vols[s]->d, ex_args[s], true, stream);
-        }
+        const int32_t n = (int32_t)vols.size();
+        hipError_t err = launch_extract_batch(d_vols, d_ex_args, n, storage, max_pool_capacity,
+                                              extract_grid, false, stream);
+        if (err == hipSuccess)
+            err = launch_extract_batch(d_vols, d_ex_args, n, storage, max_pool_capacity,
+                                       extract_grid, true, stream);
         if (timing && hipEventCreate(&e[1]) == hipSuccess) {
             hipEventRecord(e[1], stream);
             ev_pending.push_back({3, {e[0], e[1]}});
@@ -974,7 +978,7 @@ struct OsnTsdfHipBatch {
         drop_graph();
         for (auto& p : ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
         for (void* p : {(void*)d_vols, (void*)d_call_base, (void*)d_tile_prefix, (void*)d_step,
-                        (void*)d_steps, (void*)d_upd})
+                        (void*)d_steps, (void*)d_upd, (void*)d_ex_args})
             if (p) hipFree(p);
         if (stream) hipStreamDestroy(stream);
         for (auto* v : vols) {
@@ -1005,6 +1009,8 @@ OsnTsdfHipBatch* osn_tsdf_hip_batch_create(OsnTsdfHipVolume* const* volumes, int
         b->vols.push_back(v);
         hv.push_back(v->d);
         if (v->d.capacity > b->max_capacity) b->max_capacity = v->d.capacity;
+        if ((uint32_t)v->d.pool_capacity > b->max_pool_capacity)
+            b->max_pool_capacity = (uint32_t)v->d.pool_capacity;
     }
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc(&b->d_vols, sizeof(VolDev) * n) == hipSuccess &&
@@ -1148,7 +1154,11 @@ int32_t osn_tsdf_hip_batch_set_extract(OsnTsdfHipBatch* b, float* const* d_posno
         OsnTsdfHipVolume* v = b->vols[s];
         // scratch for a full pool, allocated now: nothing allocates while a
         // graph is captured or replayed
-        if (!v->grow_extract(v->d.pool_capacity)) { v->ok = false; return 2; }
+        if (!v->grow_extract(v->d.pool_capacity)) {
+            v->ok = false;
+            b->extract_every = 0;
+            return 2;
+        }
         ExtractArgs& a = b->ex_args[s];
         a.min_weight = min_weight;
         a.iso = iso;
@@ -1162,6 +1172,16 @@ int32_t osn_tsdf_hip_batch_set_extract(OsnTsdfHipBatch* b, float* const* d_posno
         a.rgb = nullptr;
         a.cap = vert_caps[s];
         a.count_out = d_counts + s;
+    }
+    if (!b->d_ex_args && hipMalloc(&b->d_ex_args, sizeof(ExtractArgs) * n) != hipSuccess) {
+        b->d_ex_args = nullptr;
+        b->extract_every = 0;
+        return 2;
+    }
+    if (hipMemcpy(b->d_ex_args, b->ex_args.data(), sizeof(ExtractArgs) * n,
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        b->extract_every = 0;
+        return 2;
     }
     return 0;
 }

@@ -25,14 +25,16 @@ namespace osn_hip {
 namespace {
 
 constexpr int kWaveThreads = 64;     // count and emit: one wave per block, many per CU
-// emit's register bound: what its LDS allows per SIMD (4-5 waves; 2 with
+// emit's register bound: what its LDS allows per SIMD (3-4 waves; 2 with
 // the colour planes of the weighted sums)
 template <int STORE>
-constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : 5; }
+constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : 4; }
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanChunk = 1024;
+constexpr int kScanGrid = 256;      // scan workgroups (grid-stride over chunks)
+constexpr uint32_t kScanGridBatch = 32;  // the same per volume of a batch
 constexpr int kTetCap = 256;          // tetrahedra listed per pass over a block
-constexpr int kStageVerts = 160;      // vertices assembled in LDS per output group
+constexpr int kStageVerts = 384;      // vertices assembled in LDS per output group (64 quads)
 
 // The 9x9x9 corner samples of one block; colour planes only where the
 // storage keeps colour (the reference's weighted sums).
@@ -218,11 +220,17 @@ __device__ inline V6 interp(float iso, P3 pa, P3 pb, float va, float vb, P3 ca, 
     return o;
 }
 
-// Triangle (a, b, c) with its face normal: positions and normals to posnor
-// from float index at_pn * 6 (the LDS staging area), colours to rgb from byte
-// index at_rgb * 3 when rgb is set.
-__device__ inline void write_tri(const V6& a, const V6& b, const V6& c, float* posnor,
-                                 int64_t at_pn, uint8_t* rgb, int64_t at_rgb) {
+// A group's vertices as assembled in LDS: positions per vertex, the face
+// normal once per triangle (vertex offsets are multiples of 3).
+struct Stage {
+    float pos[kStageVerts * 3];
+    float nor[kStageVerts];
+};
+
+// Triangle (a, b, c) with its face normal into the staging area from vertex
+// index at, colours to rgb from byte index at_rgb * 3 when rgb is set.
+__device__ inline void write_tri(const V6& a, const V6& b, const V6& c, Stage& st, int at,
+                                 uint8_t* rgb, int64_t at_rgb) {
     const float e1[3] = {b.p[0] - a.p[0], b.p[1] - a.p[1], b.p[2] - a.p[2]};
     const float e2[3] = {c.p[0] - a.p[0], c.p[1] - a.p[1], c.p[2] - a.p[2]};
     float nx = e1[1] * e2[2] - e1[2] * e2[1];
@@ -230,12 +238,12 @@ __device__ inline void write_tri(const V6& a, const V6& b, const V6& c, float* p
     float nz = e1[0] * e2[1] - e1[1] * e2[0];
     const float len = sqrtf(nx * nx + ny * ny + nz * nz);
     if (len > 1e-20f) { nx /= len; ny /= len; nz /= len; }
+    st.nor[at] = nx; st.nor[at + 1] = ny; st.nor[at + 2] = nz;
     const V6* vs[3] = {&a, &b, &c};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float* d = posnor + (size_t)(at_pn + k) * 6;
+        float* d = st.pos + (at + k) * 3;
         d[0] = vs[k]->p[0]; d[1] = vs[k]->p[1]; d[2] = vs[k]->p[2];
-        d[3] = nx; d[4] = ny; d[5] = nz;
         if (rgb) {
             uint8_t* q = rgb + (size_t)(at_rgb + k) * 3;
             q[0] = (uint8_t)fminf(fmaxf(vs[k]->c[0], 0.0f), 255.0f);
@@ -245,15 +253,20 @@ __device__ inline void write_tri(const V6& a, const V6& b, const V6& c, float* p
     }
 }
 
-// The triangles of tetrahedron t of voxel (lx, ly, lz)'s cube, written from
-// vertex index at (posnor) and at_rgb (rgb). The crossing code comes from the
+// Output float c (0-5: position, normal) of staged vertex i.
+__device__ inline float staged(const Stage& st, int i, int c) {
+    return c < 3 ? st.pos[i * 3 + c] : st.nor[i - i % 3 + (c - 3)];
+}
+
+// The triangles of tetrahedron t of voxel (lx, ly, lz)'s cube, staged from
+// vertex index at, colours written from at_rgb. The crossing code comes from the
 // tetrahedron's own 4 corners (bit j set when its j-th corner is below iso).
 // Corner positions, values and colours are formed per corner id, never held
 // in an array indexed by a lane-varying value.
 template <int STORE>
 __device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int32_t by,
                          int32_t bz, int lx, int ly, int lz, float iso, int t,
-                         float* posnor, int64_t at, uint8_t* rgb, int64_t at_rgb) {
+                         Stage& st, int at, uint8_t* rgb, int64_t at_rgb) {
     constexpr bool kCol = STORE == kStoreSums;
     const int32_t vx = bx * 8 + lx, vy = by * 8 + ly, vz = bz * 8 + lz;
     auto pos = [&](int k) {
@@ -294,14 +307,14 @@ __device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int3
         const V6 e0 = interp<kCol>(iso, pa, pb, va, vb, ca, cb);
         const V6 e1 = interp<kCol>(iso, pa, pc, va, vc, ca, c3);
         const V6 e2 = interp<kCol>(iso, pa, pd, va, vd, ca, cd);
-        write_tri(e0, e1, e2, posnor, at, rgb, at_rgb);
+        write_tri(e0, e1, e2, st, at, rgb, at_rgb);
     } else {
         const V6 ac = interp<kCol>(iso, pa, pc, va, vc, ca, c3);
         const V6 ad = interp<kCol>(iso, pa, pd, va, vd, ca, cd);
         const V6 bd = interp<kCol>(iso, pb, pd, vb, vd, cb, cd);
         const V6 bc = interp<kCol>(iso, pb, pc, vb, vc, cb, c3);
-        write_tri(ac, ad, bd, posnor, at, rgb, at_rgb);
-        write_tri(ac, bd, bc, posnor, at + 3, rgb, at_rgb + 3);
+        write_tri(ac, ad, bd, st, at, rgb, at_rgb);
+        write_tri(ac, bd, bc, st, at + 3, rgb, at_rgb + 3);
     }
 }
 
@@ -332,15 +345,17 @@ __device__ inline int wg_exclusive_scan(int x, int* s_wave, int& total) {
     return before + incl - x;
 }
 
-// Count: one wave per block, lane = row (ly, lz) of 8 cubes.
+// Count: one wave per block, lane = row (ly, lz) of 8 cubes; workgroup
+// `first` walks blocks first, first + stride, ...
 template <int STORE>
-__global__ __launch_bounds__(kWaveThreads) void k_extract_count(VolDev v, ExtractArgs a) {
+__device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t first,
+                             int32_t stride) {
     __shared__ Cube<STORE> c;
     __shared__ uint32_t s_table[256];
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
-    for (int32_t b = blockIdx.x; b < n; b += gridDim.x) {
+    for (int32_t b = first; b < n; b += stride) {
         load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb);
         uint32_t e[8];
         uint32_t n3 = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e) & 0xffffu;
@@ -351,11 +366,23 @@ __global__ __launch_bounds__(kWaveThreads) void k_extract_count(VolDev v, Extrac
     }
 }
 
-// Two-level exclusive scan of the per-block counts. Each workgroup scans a
-// 1024-block chunk; the last to finish scans the chunk totals, publishes the
-// vertex count (or -1 when it exceeds the capacity) and resets the arrival
-// counter. Workgroups never wait on each other.
-__global__ __launch_bounds__(kScanChunk) void k_extract_scan(VolDev v, ExtractArgs a) {
+template <int STORE>
+__global__ __launch_bounds__(kWaveThreads) void k_extract_count(VolDev v, ExtractArgs a) {
+    count_blocks<STORE>(v, a, blockIdx.x, gridDim.x);
+}
+
+// A batch of volumes: blockIdx.y is the volume.
+template <int STORE>
+__global__ __launch_bounds__(kWaveThreads) void k_extract_count_batch(const VolDev* vols,
+                                                                     const ExtractArgs* args) {
+    count_blocks<STORE>(vols[blockIdx.y], args[blockIdx.y], blockIdx.x, gridDim.x);
+}
+
+// Two-level exclusive scan of the per-block counts. Workgroups scan 1024-block
+// chunks (grid-stride); whoever finishes the last chunk scans the chunk
+// totals, publishes the vertex count (or -1 when it exceeds the capacity) and
+// resets the arrival counter. Workgroups never wait on each other.
+__device__ void scan_chunks(const VolDev& v, const ExtractArgs& a) {
     __shared__ int s_wave[kScanChunk / 64];
     __shared__ int s_last;
     const int32_t n = n_blocks_of(v, a);
@@ -367,17 +394,23 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan(VolDev v, ExtractAr
         }
         return;
     }
-    if ((int)blockIdx.x >= n_chunks) return;
-    const int b = blockIdx.x * kScanChunk + threadIdx.x;
-    const int x = b < n ? a.block_vcount[b] : 0;
-    int total;
-    const int ex = wg_exclusive_scan(x, s_wave, total);
-    if (b < n) a.block_voff[b] = ex;
-    if (threadIdx.x == 0) a.chunk_sums[blockIdx.x] = total;
-    __threadfence();
+    if (threadIdx.x == 0) s_last = 0;  // workgroups with no chunk
     __syncthreads();
-    if (threadIdx.x == 0) s_last = (atomicAdd(a.scan_flag, 1) == n_chunks - 1);
-    __syncthreads();
+    for (int chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
+        const int b = chunk * kScanChunk + threadIdx.x;
+        const int x = b < n ? a.block_vcount[b] : 0;
+        int total;
+        const int ex = wg_exclusive_scan(x, s_wave, total);
+        if (b < n) a.block_voff[b] = ex;
+        if (threadIdx.x == 0) a.chunk_sums[chunk] = total;
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) s_last = (atomicAdd(a.scan_flag, 1) == n_chunks - 1);
+        __syncthreads();
+        // the last arrival is this workgroup's last chunk: its later chunks
+        // would not have arrived yet
+        if (s_last) break;
+    }
     if (!s_last) return;
     __threadfence();
     // last workgroup: exclusive scan of chunk totals in place, 64-bit total
@@ -399,22 +432,31 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan(VolDev v, ExtractAr
     }
 }
 
+__global__ __launch_bounds__(kScanChunk) void k_extract_scan(VolDev v, ExtractArgs a) {
+    scan_chunks(v, a);
+}
+
+__global__ __launch_bounds__(kScanChunk) void k_extract_scan_batch(const VolDev* vols,
+                                                                  const ExtractArgs* args) {
+    scan_chunks(vols[blockIdx.y], args[blockIdx.y]);
+}
+
 // Emit: one wave per block, lane = row (ly, lz) of 8 cubes. The block's
 // tetrahedra with a crossing are listed in LDS in output order (voxel, then
 // tetrahedron), each with its vertex offset, and emitted one per lane in
 // groups whose vertices fit the staging area; each group's vertices are one
 // contiguous run of the output, copied out 8 bytes per lane.
 template <int STORE>
-__global__ __launch_bounds__(kWaveThreads, emit_waves_per_simd<STORE>()) void k_extract_emit(VolDev v, ExtractArgs a) {
+__device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first, int32_t stride) {
     __shared__ Cube<STORE> c;
     __shared__ uint32_t s_table[256];
     __shared__ uint32_t s_tets[kTetCap];
-    __shared__ float2 s_stage[kStageVerts * 3];
+    __shared__ Stage s_stage;
     if (*a.count_out < 0) return;  // overflow: write nothing
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
-    for (int32_t b = blockIdx.x; b < n; b += gridDim.x) {
+    for (int32_t b = first; b < n; b += stride) {
         if (a.block_vcount[b] == 0) continue;
         load_cube<STORE, false>(v, b, a.min_weight, c, a.block_nb);
         const int32_t bx = v.block_coord[b * 3 + 0];
@@ -465,23 +507,38 @@ __global__ __launch_bounds__(kWaveThreads, emit_waves_per_simd<STORE>()) void k_
                 if (lane < cnt) {
                     const int vi = (int)(ent >> 19), t = (int)((ent >> 16) & 7u);
                     emit_tet(c, v, bx, by, bz, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, t,
-                             reinterpret_cast<float*>(s_stage), vo - v_lo, a.rgb, base + vo);
+                             s_stage, vo - v_lo, a.rgb, base + vo);
                 }
                 const int n_verts = __shfl(rel_end, cnt - 1);
                 __syncthreads();
                 float* out = a.posnor + (base + v_lo) * 6;
                 if ((reinterpret_cast<uintptr_t>(out) & 7u) == 0) {
+                    // float2 j: vertex j / 3, floats 2 (j % 3) and 2 (j % 3) + 1
                     float2* dst = reinterpret_cast<float2*>(out);
-                    for (int j = lane; j < n_verts * 3; j += kWaveThreads) dst[j] = s_stage[j];
+                    for (int j = lane; j < n_verts * 3; j += kWaveThreads) {
+                        const int i = j / 3, c = 2 * (j - 3 * i);
+                        dst[j] = make_float2(staged(s_stage, i, c), staged(s_stage, i, c + 1));
+                    }
                 } else {  // a caller's buffer need only be float-aligned
-                    const float* src = reinterpret_cast<const float*>(s_stage);
-                    for (int j = lane; j < n_verts * 6; j += kWaveThreads) out[j] = src[j];
+                    for (int j = lane; j < n_verts * 6; j += kWaveThreads)
+                        out[j] = staged(s_stage, j / 6, j % 6);
                 }
                 __syncthreads();
                 g += cnt;
             }
         }
     }
+}
+
+template <int STORE>
+__global__ __launch_bounds__(kWaveThreads, emit_waves_per_simd<STORE>()) void k_extract_emit(VolDev v, ExtractArgs a) {
+    emit_blocks<STORE>(v, a, blockIdx.x, gridDim.x);
+}
+
+template <int STORE>
+__global__ __launch_bounds__(kWaveThreads, emit_waves_per_simd<STORE>()) void k_extract_emit_batch(
+    const VolDev* vols, const ExtractArgs* args) {
+    emit_blocks<STORE>(vols[blockIdx.y], args[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
 }  // namespace
@@ -492,8 +549,8 @@ hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipS
     const int32_t grid_blocks = a.n_blocks >= 0 ? (a.n_blocks > 0 ? a.n_blocks : 1)
                                                 : (v.pool_capacity < 32768 ? v.pool_capacity : 32768);
     const int32_t n_for_scan = a.n_blocks >= 0 ? a.n_blocks : v.pool_capacity;
-    const int32_t scan_grid = (n_for_scan + kScanChunk - 1) / kScanChunk > 0
-                                  ? (n_for_scan + kScanChunk - 1) / kScanChunk : 1;
+    const int32_t n_chunks = (n_for_scan + kScanChunk - 1) / kScanChunk;
+    const int32_t scan_grid = n_chunks < 1 ? 1 : (n_chunks < kScanGrid ? n_chunks : kScanGrid);
     if (!emit) {
         switch (v.storage) {
             case kStoreSums: k_extract_count<kStoreSums><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
@@ -509,6 +566,33 @@ hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipS
             case kStoreF32: k_extract_emit<kStoreF32><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
             case kStoreF16: k_extract_emit<kStoreF16><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
             case kStoreF64: k_extract_emit<kStoreF64><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
+            default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_extract_batch(const VolDev* vols, const ExtractArgs* args, int32_t n_scenes,
+                                int32_t storage, uint32_t max_pool_capacity,
+                                int32_t grid_per_scene, bool emit, hipStream_t s) {
+    if (n_scenes <= 0 || grid_per_scene <= 0) return hipErrorInvalidValue;
+    const uint32_t cap = max_pool_capacity > 0 ? max_pool_capacity : 1;
+    const dim3 grid(cap < (uint32_t)grid_per_scene ? cap : (uint32_t)grid_per_scene, n_scenes);
+    const uint32_t n_chunks = (cap + kScanChunk - 1) / kScanChunk;
+    const dim3 scan_grid(n_chunks < kScanGridBatch ? n_chunks : kScanGridBatch, n_scenes);
+    if (!emit) {
+        switch (storage) {
+            case kStoreF32: k_extract_count_batch<kStoreF32><<<grid, kWaveThreads, 0, s>>>(vols, args); break;
+            case kStoreF16: k_extract_count_batch<kStoreF16><<<grid, kWaveThreads, 0, s>>>(vols, args); break;
+            case kStoreF64: k_extract_count_batch<kStoreF64><<<grid, kWaveThreads, 0, s>>>(vols, args); break;
+            default: return hipErrorInvalidValue;
+        }
+        k_extract_scan_batch<<<scan_grid, kScanChunk, 0, s>>>(vols, args);
+    } else {
+        switch (storage) {
+            case kStoreF32: k_extract_emit_batch<kStoreF32><<<grid, kWaveThreads, 0, s>>>(vols, args); break;
+            case kStoreF16: k_extract_emit_batch<kStoreF16><<<grid, kWaveThreads, 0, s>>>(vols, args); break;
+            case kStoreF64: k_extract_emit_batch<kStoreF64><<<grid, kWaveThreads, 0, s>>>(vols, args); break;
             default: return hipErrorInvalidValue;
         }
     }

@@ -138,6 +138,13 @@ hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t 
                                     const LaunchEvents& ev = {});
 hipError_t launch_step_advance(int32_t* step_ctr, hipStream_t s);
 hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipStream_t s);
+// Count + scan (emit false) or emit for every volume of a batch, one launch
+// each: blockIdx.y is the volume, grid_per_scene one-wave workgroups walk its
+// blocks (fewer when the pools are smaller).
+constexpr int32_t kExtractGridPerScene = 8192;
+hipError_t launch_extract_batch(const VolDev* vols, const ExtractArgs* args, int32_t n_scenes,
+                                int32_t storage, uint32_t max_pool_capacity,
+                                int32_t grid_per_scene, bool emit, hipStream_t s);
 hipError_t launch_fill_table(HashEntry* table, uint32_t n, hipStream_t s);
 hipError_t launch_cas_ceiling(unsigned long long* slots, uint32_t mask, int32_t n_ops,
                               hipStream_t s);
