@@ -32,6 +32,7 @@ constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : 4; }
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanChunk = 1024;
 constexpr int kScanGrid = 256;      // scan workgroups (grid-stride over chunks)
+constexpr int kSerialChunks = 64;   // up to this many chunks one workgroup scans alone
 constexpr uint32_t kScanGridBatch = 32;  // the same per volume of a batch
 constexpr int kTetCap = 256;          // tetrahedra listed per pass over a block
 constexpr int kStageVerts = 384;      // vertices assembled in LDS per output group (64 quads)
@@ -378,9 +379,10 @@ __global__ __launch_bounds__(kWaveThreads) void k_extract_count_batch(const VolD
     count_blocks<STORE>(vols[blockIdx.y], args[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
-// Two-level exclusive scan of the per-block counts. Workgroups scan 1024-block
-// chunks (grid-stride); whoever finishes the last chunk scans the chunk
-// totals, publishes the vertex count (or -1 when it exceeds the capacity) and
+// Exclusive scan of the per-block counts; publishes the vertex count (or -1
+// when it exceeds the capacity). Up to kSerialChunks chunks of 1024 blocks
+// one workgroup scans them all. Above that, workgroups scan 1024-block chunks
+// (grid-stride); whoever finishes the last chunk scans the chunk totals and
 // resets the arrival counter. Workgroups never wait on each other.
 __device__ void scan_chunks(const VolDev& v, const ExtractArgs& a) {
     __shared__ int s_wave[kScanChunk / 64];
@@ -391,6 +393,30 @@ __device__ void scan_chunks(const VolDev& v, const ExtractArgs& a) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             *a.count_out = 0;
             v.ctr->mesh_vertices = 0;
+        }
+        return;
+    }
+    if (n_chunks <= kSerialChunks) {
+        // Small volumes: workgroup 0 alone, each thread a run of up to
+        // kSerialChunks consecutive blocks; offsets are absolute and the
+        // chunk sums zero. No fences, no arrival counter.
+        if (blockIdx.x != 0) return;
+        const int first = threadIdx.x * n_chunks;
+        int sum = 0;
+        for (int k = 0; k < n_chunks; ++k)
+            if (first + k < n) sum += a.block_vcount[first + k];
+        int total;  // <= 64 x 1024 blocks x 18432 vertices < 2^31
+        int run = wg_exclusive_scan(sum, s_wave, total);
+        for (int k = 0; k < n_chunks; ++k)
+            if (first + k < n) {
+                a.block_voff[first + k] = run;
+                run += a.block_vcount[first + k];
+            }
+        if (threadIdx.x < n_chunks) a.chunk_sums[threadIdx.x] = 0;
+        if (threadIdx.x == 0) {
+            const int32_t out = total > a.cap ? -1 : total;
+            *a.count_out = out;
+            v.ctr->mesh_vertices = out;
         }
         return;
     }
@@ -513,11 +539,15 @@ __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first
                 __syncthreads();
                 float* out = a.posnor + (base + v_lo) * 6;
                 if ((reinterpret_cast<uintptr_t>(out) & 7u) == 0) {
-                    // float2 j: vertex j / 3, floats 2 (j % 3) and 2 (j % 3) + 1
+                    // one vertex per lane: three float2 stores, so each
+                    // wave-wide store covers every third 8 bytes of one run
                     float2* dst = reinterpret_cast<float2*>(out);
-                    for (int j = lane; j < n_verts * 3; j += kWaveThreads) {
-                        const int i = j / 3, c = 2 * (j - 3 * i);
-                        dst[j] = make_float2(staged(s_stage, i, c), staged(s_stage, i, c + 1));
+                    for (int i = lane; i < n_verts; i += kWaveThreads) {
+                        const float* p = s_stage.pos + i * 3;
+                        const float* q = s_stage.nor + (i - i % 3);
+                        dst[i * 3 + 0] = make_float2(p[0], p[1]);
+                        dst[i * 3 + 1] = make_float2(p[2], q[0]);
+                        dst[i * 3 + 2] = make_float2(q[1], q[2]);
                     }
                 } else {  // a caller's buffer need only be float-aligned
                     for (int j = lane; j < n_verts * 6; j += kWaveThreads)
