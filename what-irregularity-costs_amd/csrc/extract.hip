@@ -25,24 +25,31 @@ namespace osn_hip {
 namespace {
 
 constexpr int kWaveThreads = 64;     // count and emit: one wave per block, many per CU
-// emit's register bound: what its LDS allows per SIMD (3-4 waves; 2 with
-// the colour planes of the weighted sums)
+// emit's register bound: what its LDS allows per SIMD (5 waves with FP16
+// corners, 3-4 with FP32, 2 with the colour planes of the weighted sums)
 template <int STORE>
-constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : 4; }
+constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : (STORE == kStoreF16 ? 5 : 4); }
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanChunk = 1024;
 constexpr int kScanGrid = 256;      // scan workgroups (grid-stride over chunks)
 constexpr int kSerialChunks = 64;   // up to this many chunks one workgroup scans alone
 constexpr uint32_t kScanGridBatch = 32;  // the same per volume of a batch
 constexpr int kTetCap = 256;          // tetrahedra listed per pass over a block
-constexpr int kStageVerts = 384;      // vertices assembled in LDS per output group (64 quads)
+constexpr int kStageVerts = 288;      // vertices assembled in LDS per output group (measured best of 224-384)
 
 // The 9x9x9 corner samples of one block; colour planes only where the
-// storage keeps colour (the reference's weighted sums).
+// storage keeps colour (the reference's weighted sums). The FP16 storage's
+// mean tsdf is a half, kept as one in LDS (exact; NaN marks an invalid
+// corner as for the others).
 template <int STORE>
 struct Cube {
     float val[kCube];     // mean tsdf, NaN when the corner is not a valid sample
     int32_t nb[8];        // pool index of the block at offset (x, y, z) bits
+};
+template <>
+struct Cube<kStoreF16> {
+    unsigned short val[kCube];  // half bits
+    int32_t nb[8];
 };
 template <>
 struct Cube<kStoreSums> {
@@ -50,6 +57,9 @@ struct Cube<kStoreSums> {
     int32_t nb[8];
     float col[3][kCube];  // mean colour
 };
+
+__device__ inline float corner(float x) { return x; }
+__device__ inline float corner(unsigned short h) { return __half2float(__ushort_as_half(h)); }
 
 __device__ inline int cube_index(int x, int y, int z) { return (z * 9 + y) * 9 + x; }
 
@@ -81,6 +91,7 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
         const int o = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
         const int32_t nbi = c.nb[o];
         float t = __builtin_nanf("");
+        unsigned short h = 0x7e00u;  // half NaN
         float cr = 0.f, cg = 0.f, cb = 0.f;
         if (nbi >= 0) {
             const size_t vi = (size_t)nbi * kBlockVoxels + (((z & 7) * 8 + (y & 7)) * 8 + (x & 7));
@@ -99,17 +110,15 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
             } else if (STORE == kStoreF16) {
                 const uint32_t q = static_cast<const uint32_t*>(v.vox)[vi];
                 const float w = (float)(q >> 16);
-                if (!(w < min_w || w <= 0.0f)) {
-                    t = __half2float(__ushort_as_half((unsigned short)(q & 0xffffu)));
-                    cr = cg = cb = 128.0f;
-                }
+                if (!(w < min_w || w <= 0.0f)) h = (unsigned short)(q & 0xffffu);
             } else {
                 const double2 q = static_cast<const double2*>(v.vox)[vi];
                 const float w = (float)q.y;
                 if (!(w < min_w || w <= 0.0f)) { t = (float)q.x; cr = cg = cb = 128.0f; }
             }
         }
-        c.val[i] = t;
+        if constexpr (STORE == kStoreF16) c.val[i] = h;
+        else c.val[i] = t;
         if constexpr (STORE == kStoreSums) {
             c.col[0][i] = cr; c.col[1][i] = cg; c.col[2][i] = cb;
         }
@@ -168,10 +177,10 @@ __device__ inline uint32_t classify_row(const Cube<STORE>& c, const uint32_t* ta
     for (int x = 0; x < 9; ++x) {
         // corners (x, ly + dy, lz + dz) at mask bit 2 dy + 4 dz: the cube's
         // corner ids with dx = 0
-        const float p = c.val[cube_index(x, ly, lz)];
-        const float q = c.val[cube_index(x, ly + 1, lz)];
-        const float r = c.val[cube_index(x, ly, lz + 1)];
-        const float s = c.val[cube_index(x, ly + 1, lz + 1)];
+        const float p = corner(c.val[cube_index(x, ly, lz)]);
+        const float q = corner(c.val[cube_index(x, ly + 1, lz)]);
+        const float r = corner(c.val[cube_index(x, ly, lz + 1)]);
+        const float s = corner(c.val[cube_index(x, ly + 1, lz + 1)]);
         const bool ok = !(__builtin_isnan(p) || __builtin_isnan(q) || __builtin_isnan(r) ||
                           __builtin_isnan(s));
         const uint32_t bits = (p < iso ? 1u : 0u) | (q < iso ? 4u : 0u) | (r < iso ? 16u : 0u) |
@@ -276,7 +285,7 @@ __device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int3
                   ((float)(vz + ((k >> 2) & 1)) + 0.5f) * v.voxel};
     };
     auto ci = [&](int k) { return cube_index(lx + (k & 1), ly + ((k >> 1) & 1), lz + ((k >> 2) & 1)); };
-    auto val = [&](int k) { return c.val[ci(k)]; };
+    auto val = [&](int k) { return corner(c.val[ci(k)]); };
     auto col = [&](int k) {
         if constexpr (kCol) return P3{c.col[0][ci(k)], c.col[1][ci(k)], c.col[2][ci(k)]};
         else return P3{128.0f, 128.0f, 128.0f};
