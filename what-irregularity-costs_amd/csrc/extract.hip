@@ -25,10 +25,11 @@ namespace osn_hip {
 namespace {
 
 constexpr int kWaveThreads = 64;     // count and emit: one wave per block, many per CU
-// emit's register bound: what its LDS allows per SIMD (5 waves with FP16
-// corners, 3-4 with FP32, 2 with the colour planes of the weighted sums)
+// emit's register bound: 4 waves per SIMD (128 VGPRs, no spills; its LDS
+// allows 5 with FP16 corners, 3-4 with FP32), 2 with the colour planes of
+// the weighted sums
 template <int STORE>
-constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : (STORE == kStoreF16 ? 5 : 4); }
+constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : 4; }
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanChunk = 1024;
 constexpr int kScanGrid = 256;      // scan workgroups (grid-stride over chunks)
@@ -58,6 +59,14 @@ struct Cube<kStoreSums> {
     float col[3][kCube];  // mean colour
 };
 
+// A voxel record as the packed storages keep it.
+template <int STORE> struct RawVox;
+template <> struct RawVox<kStoreF32> { using T = float2; };   // tsdf, weight
+template <> struct RawVox<kStoreF16> { using T = uint32_t; }; // half tsdf | u16 weight << 16
+template <> struct RawVox<kStoreF64> { using T = double2; };  // tsdf, weight
+
+constexpr int kCubeIters = (kCube + kWaveThreads - 1) / kWaveThreads;  // corners per lane
+
 __device__ inline float corner(float x) { return x; }
 __device__ inline float corner(unsigned short h) { return __half2float(__ushort_as_half(h)); }
 
@@ -86,16 +95,47 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
         c.nb[tid] = nbi;
     }
     __syncthreads();
-    for (int i = tid; i < kCube; i += blockDim.x) {
-        const int x = i % 9, y = (i / 9) % 9, z = i / 81;
-        const int o = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
-        const int32_t nbi = c.nb[o];
-        float t = __builtin_nanf("");
-        unsigned short h = 0x7e00u;  // half NaN
-        float cr = 0.f, cg = 0.f, cb = 0.f;
-        if (nbi >= 0) {
-            const size_t vi = (size_t)nbi * kBlockVoxels + (((z & 7) * 8 + (y & 7)) * 8 + (x & 7));
-            if (STORE == kStoreSums) {
+    if constexpr (STORE != kStoreSums) {
+        // every lane issues all its loads before using any: kCubeIters
+        // loads in flight, one wait. A corner of an absent block reads its
+        // own block (in bounds) and is then marked invalid.
+        using T = typename RawVox<STORE>::T;
+        const T* vox = static_cast<const T*>(v.vox);
+        T q[kCubeIters];
+        bool present[kCubeIters];
+#pragma unroll
+        for (int k = 0; k < kCubeIters; ++k) {
+            const int i = tid + k * kWaveThreads;
+            const int x = i % 9, y = (i / 9) % 9, z = i / 81;
+            const int o = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
+            const int32_t nbi = i < kCube ? c.nb[o] : -1;
+            present[k] = nbi >= 0;
+            q[k] = vox[(size_t)(present[k] ? nbi : bi) * kBlockVoxels +
+                       (((z & 7) * 8 + (y & 7)) * 8 + (x & 7))];
+        }
+#pragma unroll
+        for (int k = 0; k < kCubeIters; ++k) {
+            const int i = tid + k * kWaveThreads;
+            if (i >= kCube) break;
+            if constexpr (STORE == kStoreF16) {
+                const float w = (float)(q[k] >> 16);
+                c.val[i] = present[k] && !(w < min_w || w <= 0.0f) ? (unsigned short)(q[k] & 0xffffu)
+                                                                    : (unsigned short)0x7e00u;
+            } else {
+                const float w = (float)q[k].y;
+                c.val[i] = present[k] && !(w < min_w || w <= 0.0f) ? (float)q[k].x
+                                                                    : __builtin_nanf("");
+            }
+        }
+    } else {
+        for (int i = tid; i < kCube; i += kWaveThreads) {
+            const int x = i % 9, y = (i / 9) % 9, z = i / 81;
+            const int o = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
+            const int32_t nbi = c.nb[o];
+            float t = __builtin_nanf("");
+            float cr = 0.f, cg = 0.f, cb = 0.f;
+            if (nbi >= 0) {
+                const size_t vi = (size_t)nbi * kBlockVoxels + (((z & 7) * 8 + (y & 7)) * 8 + (x & 7));
                 const float w = v.weight[vi];
                 if (!(w < min_w || w <= 0.0f)) {
                     const float inv_w = 1.0f / w;
@@ -104,22 +144,8 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
                     cg = v.g[vi] * inv_w;
                     cb = v.b[vi] * inv_w;
                 }
-            } else if (STORE == kStoreF32) {
-                const float2 q = static_cast<const float2*>(v.vox)[vi];
-                if (!(q.y < min_w || q.y <= 0.0f)) { t = q.x; cr = cg = cb = 128.0f; }
-            } else if (STORE == kStoreF16) {
-                const uint32_t q = static_cast<const uint32_t*>(v.vox)[vi];
-                const float w = (float)(q >> 16);
-                if (!(w < min_w || w <= 0.0f)) h = (unsigned short)(q & 0xffffu);
-            } else {
-                const double2 q = static_cast<const double2*>(v.vox)[vi];
-                const float w = (float)q.y;
-                if (!(w < min_w || w <= 0.0f)) { t = (float)q.x; cr = cg = cb = 128.0f; }
             }
-        }
-        if constexpr (STORE == kStoreF16) c.val[i] = h;
-        else c.val[i] = t;
-        if constexpr (STORE == kStoreSums) {
+            c.val[i] = t;
             c.col[0][i] = cr; c.col[1][i] = cg; c.col[2][i] = cb;
         }
     }
@@ -383,8 +409,8 @@ __global__ __launch_bounds__(kWaveThreads) void k_extract_count(VolDev v, Extrac
 
 // A batch of volumes: blockIdx.y is the volume.
 template <int STORE>
-__global__ __launch_bounds__(kWaveThreads) void k_extract_count_batch(const VolDev* vols,
-                                                                     const ExtractArgs* args) {
+__global__ __launch_bounds__(kWaveThreads) void k_extract_count_batch(
+    const VolDev* __restrict__ vols, const ExtractArgs* __restrict__ args) {
     count_blocks<STORE>(vols[blockIdx.y], args[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
@@ -471,8 +497,8 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan(VolDev v, ExtractAr
     scan_chunks(v, a);
 }
 
-__global__ __launch_bounds__(kScanChunk) void k_extract_scan_batch(const VolDev* vols,
-                                                                  const ExtractArgs* args) {
+__global__ __launch_bounds__(kScanChunk) void k_extract_scan_batch(
+    const VolDev* __restrict__ vols, const ExtractArgs* __restrict__ args) {
     scan_chunks(vols[blockIdx.y], args[blockIdx.y]);
 }
 
@@ -501,7 +527,7 @@ __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first
         uint32_t e[8];
         const uint32_t row = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e);
         const uint32_t incl = wave_inclusive_scan(row);
-        const int n_tets = (int)(__shfl(incl, 63) >> 16);
+        const int n_tets = (int)((uint32_t)__builtin_amdgcn_readlane((int)incl, 63) >> 16);
         const uint32_t before = incl - row;
         // passes of up to kTetCap listed tetrahedra (one pass unless the
         // block's surface is very dense)
@@ -533,7 +559,7 @@ __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first
                 const int i = g + lane;
                 const uint32_t ent = i < n_pass ? s_tets[i] : 0u;
                 const int vo = (int)(ent & 0x7fffu);
-                const int v_lo = __shfl(vo, 0);
+                const int v_lo = __builtin_amdgcn_readfirstlane(vo);  // lane 0: tetrahedron g
                 const int rel_end = vo + (((ent >> 15) & 1u) ? 6 : 3) - v_lo;
                 // the group: the longest run of listed tetrahedra whose
                 // vertices fit the staging area (at least one)
@@ -544,7 +570,7 @@ __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first
                     emit_tet(c, v, bx, by, bz, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, t,
                              s_stage, vo - v_lo, a.rgb, base + vo);
                 }
-                const int n_verts = __shfl(rel_end, cnt - 1);
+                const int n_verts = __builtin_amdgcn_readlane(rel_end, cnt - 1);
                 __syncthreads();
                 float* out = a.posnor + (base + v_lo) * 6;
                 if ((reinterpret_cast<uintptr_t>(out) & 7u) == 0) {
@@ -576,7 +602,7 @@ __global__ __launch_bounds__(kWaveThreads, emit_waves_per_simd<STORE>()) void k_
 
 template <int STORE>
 __global__ __launch_bounds__(kWaveThreads, emit_waves_per_simd<STORE>()) void k_extract_emit_batch(
-    const VolDev* vols, const ExtractArgs* args) {
+    const VolDev* __restrict__ vols, const ExtractArgs* __restrict__ args) {
     emit_blocks<STORE>(vols[blockIdx.y], args[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
