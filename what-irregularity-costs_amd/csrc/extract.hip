@@ -235,27 +235,6 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
 struct P3 { float x, y, z; };
 struct V6 { float p[3]; float c[3]; };
 
-template <bool COL>
-__device__ inline V6 interp(float iso, P3 pa, P3 pb, float va, float vb, P3 ca, P3 cb) {
-    const bool sw = (pb.x < pa.x) || (pb.x == pa.x && pb.y < pa.y) ||
-                    (pb.x == pa.x && pb.y == pa.y && pb.z < pa.z);
-    const P3 p0 = sw ? pb : pa, p1 = sw ? pa : pb;
-    const P3 c0 = sw ? cb : ca, c1 = sw ? ca : cb;
-    const float v0 = sw ? vb : va;
-    const float v1 = sw ? va : vb;
-    const float den = v1 - v0;
-    float t = fabsf(den) < 1e-12f ? 0.5f : (iso - v0) / den;
-    t = fminf(fmaxf(t, 0.0f), 1.0f);
-    V6 o;
-    o.p[0] = p0.x + t * (p1.x - p0.x);
-    o.p[1] = p0.y + t * (p1.y - p0.y);
-    o.p[2] = p0.z + t * (p1.z - p0.z);
-    o.c[0] = COL ? c0.x + t * (c1.x - c0.x) : 128.0f;
-    o.c[1] = COL ? c0.y + t * (c1.y - c0.y) : 128.0f;
-    o.c[2] = COL ? c0.z + t * (c1.z - c0.z) : 128.0f;
-    return o;
-}
-
 // A group's vertices as assembled in LDS: positions per vertex, the face
 // normal once per triangle (vertex offsets are multiples of 3).
 struct Stage {
@@ -294,63 +273,79 @@ __device__ inline float staged(const Stage& st, int i, int c) {
     return c < 3 ? st.pos[i * 3 + c] : st.nor[i - i % 3 + (c - 3)];
 }
 
+// Corner k's rank in the lexicographic order of positions (x, then y, then
+// z): the order in which an edge's two endpoints are taken.
+__device__ inline int lex_key(int k) { return ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1); }
+
+// Per crossing code (bit j set when the tetrahedron's j-th corner is below
+// iso) the roles (a, b, c, d) as positions 0-3 among its corners, 2 bits
+// each, a lowest; 0 for the codes without a crossing (0 and 15). Codes with
+// one or three corners below iso cut triangle (ab, ac, ad), codes with two a
+// quad split into (ac, ad, bd) and (ac, bd, bc).
+constexpr unsigned long long kRolesLo = 0x93C9D8D2E4E1E400ull;  // codes 0-7
+constexpr unsigned long long kRolesHi = 0x00E4E14ED28D9C93ull;  // codes 8-15
+
 // The triangles of tetrahedron t of voxel (lx, ly, lz)'s cube, staged from
-// vertex index at, colours written from at_rgb. The crossing code comes from the
-// tetrahedron's own 4 corners (bit j set when its j-th corner is below iso).
-// Corner positions, values and colours are formed per corner id, never held
-// in an array indexed by a lane-varying value.
+// vertex index at, colours written from at_rgb. One path for triangles and
+// quads: three edge interpolations, a fourth and the second triangle for
+// quads only. Corner positions, values and colours are formed per corner id,
+// never held in an array indexed by a lane-varying value.
 template <int STORE>
 __device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int32_t by,
                          int32_t bz, int lx, int ly, int lz, float iso, int t,
                          Stage& st, int at, uint8_t* rgb, int64_t at_rgb) {
     constexpr bool kCol = STORE == kStoreSums;
     const int32_t vx = bx * 8 + lx, vy = by * 8 + ly, vz = bz * 8 + lz;
-    auto pos = [&](int k) {
-        return P3{((float)(vx + (k & 1)) + 0.5f) * v.voxel,
-                  ((float)(vy + ((k >> 1) & 1)) + 0.5f) * v.voxel,
-                  ((float)(vz + ((k >> 2) & 1)) + 0.5f) * v.voxel};
-    };
+    // each axis' two corner coordinates (offset 0 and 1)
+    const float x0 = ((float)vx + 0.5f) * v.voxel, x1 = ((float)(vx + 1) + 0.5f) * v.voxel;
+    const float y0 = ((float)vy + 0.5f) * v.voxel, y1 = ((float)(vy + 1) + 0.5f) * v.voxel;
+    const float z0 = ((float)vz + 0.5f) * v.voxel, z1 = ((float)(vz + 1) + 0.5f) * v.voxel;
     auto ci = [&](int k) { return cube_index(lx + (k & 1), ly + ((k >> 1) & 1), lz + ((k >> 2) & 1)); };
     auto val = [&](int k) { return corner(c.val[ci(k)]); };
-    auto col = [&](int k) {
-        if constexpr (kCol) return P3{c.col[0][ci(k)], c.col[1][ci(k)], c.col[2][ci(k)]};
-        else return P3{128.0f, 128.0f, 128.0f};
+    // the point of edge (ka, kb) where the mean tsdf crosses iso, endpoints in
+    // position order (cpp/src/marching_tets.cpp:37-80)
+    auto interp = [&](int ka, int kb) {
+        const bool sw = lex_key(kb) < lex_key(ka);
+        const int k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+        const float v0 = val(k0), v1 = val(k1);
+        const float den = v1 - v0;
+        float w = fabsf(den) < 1e-12f ? 0.5f : (iso - v0) / den;
+        w = fminf(fmaxf(w, 0.0f), 1.0f);
+        const float px0 = (k0 & 1) ? x1 : x0, px1 = (k1 & 1) ? x1 : x0;
+        const float py0 = (k0 & 2) ? y1 : y0, py1 = (k1 & 2) ? y1 : y0;
+        const float pz0 = (k0 & 4) ? z1 : z0, pz1 = (k1 & 4) ? z1 : z0;
+        V6 o;
+        o.p[0] = px0 + w * (px1 - px0);
+        o.p[1] = py0 + w * (py1 - py0);
+        o.p[2] = pz0 + w * (pz1 - pz0);
+        if constexpr (kCol) {
+            const int i0 = ci(k0), i1 = ci(k1);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float c0 = c.col[q][i0], c1 = c.col[q][i1];
+                o.c[q] = c0 + w * (c1 - c0);
+            }
+        } else {
+            o.c[0] = o.c[1] = o.c[2] = 128.0f;
+        }
+        return o;
     };
     const uint32_t tc = tet_corners(t);
-    const int i0 = (int)(tc & 15u), i1 = (int)((tc >> 4) & 15u), i2 = (int)((tc >> 8) & 15u),
-              i3 = (int)(tc >> 12);
-    const int code = (val(i0) < iso ? 1 : 0) | (val(i1) < iso ? 2 : 0) |
-                     (val(i2) < iso ? 4 : 0) | (val(i3) < iso ? 8 : 0);
-    int a, b, cc, d;
-    bool quad;
-    switch (code) {
-        case 0x01: case 0x0E: a = i0; b = i1; cc = i2; d = i3; quad = false; break;
-        case 0x02: case 0x0D: a = i1; b = i0; cc = i2; d = i3; quad = false; break;
-        case 0x04: case 0x0B: a = i2; b = i0; cc = i1; d = i3; quad = false; break;
-        case 0x08: case 0x07: a = i3; b = i0; cc = i1; d = i2; quad = false; break;
-        case 0x03: a = i0; b = i1; cc = i2; d = i3; quad = true; break;
-        case 0x0C: a = i2; b = i3; cc = i0; d = i1; quad = true; break;
-        case 0x05: a = i0; b = i2; cc = i1; d = i3; quad = true; break;
-        case 0x0A: a = i1; b = i3; cc = i0; d = i2; quad = true; break;
-        case 0x06: a = i1; b = i2; cc = i0; d = i3; quad = true; break;
-        case 0x09: a = i0; b = i3; cc = i1; d = i2; quad = true; break;
-        default: return;
-    }
-    const P3 pa = pos(a), pb = pos(b), pc = pos(cc), pd = pos(d);
-    const float va = val(a), vb = val(b), vc = val(cc), vd = val(d);
-    const P3 ca = col(a), cb = col(b), c3 = col(cc), cd = col(d);
-    if (!quad) {
-        const V6 e0 = interp<kCol>(iso, pa, pb, va, vb, ca, cb);
-        const V6 e1 = interp<kCol>(iso, pa, pc, va, vc, ca, c3);
-        const V6 e2 = interp<kCol>(iso, pa, pd, va, vd, ca, cd);
-        write_tri(e0, e1, e2, st, at, rgb, at_rgb);
-    } else {
-        const V6 ac = interp<kCol>(iso, pa, pc, va, vc, ca, c3);
-        const V6 ad = interp<kCol>(iso, pa, pd, va, vd, ca, cd);
-        const V6 bd = interp<kCol>(iso, pb, pd, vb, vd, cb, cd);
-        const V6 bc = interp<kCol>(iso, pb, pc, vb, vc, cb, c3);
-        write_tri(ac, ad, bd, st, at, rgb, at_rgb);
-        write_tri(ac, bd, bc, st, at + 3, rgb, at_rgb + 3);
+    const int code = (val((int)(tc & 15u)) < iso ? 1 : 0) | (val((int)((tc >> 4) & 15u)) < iso ? 2 : 0) |
+                     (val((int)((tc >> 8) & 15u)) < iso ? 4 : 0) | (val((int)(tc >> 12)) < iso ? 8 : 0);
+    const uint32_t roles =
+        (uint32_t)((code < 8 ? kRolesLo >> (8 * code) : kRolesHi >> (8 * (code - 8))) & 0xffu);
+    if (roles == 0u) return;
+    auto role = [&](int j) { return (int)((tc >> (4u * ((roles >> (2 * j)) & 3u))) & 15u); };
+    const int a = role(0), b = role(1), cc = role(2), d = role(3);
+    const bool quad = __popc((uint32_t)code) == 2;
+    const V6 e0 = interp(a, quad ? cc : b);
+    const V6 e1 = interp(a, quad ? d : cc);
+    const V6 e2 = interp(quad ? b : a, d);
+    write_tri(e0, e1, e2, st, at, rgb, at_rgb);
+    if (quad) {
+        const V6 e3 = interp(b, cc);
+        write_tri(e0, e2, e3, st, at + 3, rgb, at_rgb + 3);
     }
 }
 
