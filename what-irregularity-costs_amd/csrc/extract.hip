@@ -25,11 +25,11 @@ namespace osn_hip {
 namespace {
 
 constexpr int kWaveThreads = 64;     // count and emit: one wave per block, many per CU
-// emit's register bound: 4 waves per SIMD (128 VGPRs, no spills; its LDS
-// allows 5 with FP16 corners, 3-4 with FP32), 2 with the colour planes of
-// the weighted sums
+// emit's register bound: 5 waves per SIMD with FP16 corners (96 VGPRs, what
+// its LDS allows), 4 with FP32 (128), 2 with the colour planes of the
+// weighted sums
 template <int STORE>
-constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : 4; }
+constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : (STORE == kStoreF16 ? 5 : 4); }
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanChunk = 1024;
 constexpr int kScanGrid = 256;      // scan workgroups (grid-stride over chunks)
@@ -65,12 +65,18 @@ template <> struct RawVox<kStoreF32> { using T = float2; };   // tsdf, weight
 template <> struct RawVox<kStoreF16> { using T = uint32_t; }; // half tsdf | u16 weight << 16
 template <> struct RawVox<kStoreF64> { using T = double2; };  // tsdf, weight
 
-constexpr int kCubeIters = (kCube + kWaveThreads - 1) / kWaveThreads;  // corners per lane
-
 __device__ inline float corner(float x) { return x; }
 __device__ inline float corner(unsigned short h) { return __half2float(__ushort_as_half(h)); }
 
 __device__ inline int cube_index(int x, int y, int z) { return (z * 9 + y) * 9 + x; }
+
+// The FP16 storage's u16 weight w passes !(w < min_w || w <= 0) exactly when
+// w >= this floor (w is an integer; a NaN min_w rejects only w = 0).
+__device__ inline uint32_t f16_weight_floor(float min_w) {
+    if (!(min_w > 1.0f)) return 1u;
+    if (min_w > 65535.0f) return 65536u;
+    return (uint32_t)ceilf(min_w);
+}
 
 // Stage block bi's corner samples. The count pass (LOOKUP) finds the 7
 // neighbour blocks in the table and records their indices in nb_out[bi * 8];
@@ -96,35 +102,50 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
     }
     __syncthreads();
     if constexpr (STORE != kStoreSums) {
-        // every lane issues all its loads before using any: kCubeIters
-        // loads in flight, one wait. A corner of an absent block reads its
-        // own block (in bounds) and is then marked invalid.
+        // Rows of 9 corners (y, z fixed): row `tid` and, for tid < 17, row
+        // tid + 64. A row's corners x < 8 lie in one block, x = 8 in its +x
+        // neighbour: two base addresses, immediate offsets. Every lane issues
+        // all its loads before using any; a corner of an absent block reads
+        // its own block (in bounds) and is then marked invalid.
         using T = typename RawVox<STORE>::T;
         const T* vox = static_cast<const T*>(v.vox);
-        T q[kCubeIters];
-        bool present[kCubeIters];
+        constexpr int kRows = 81;
+        T q[2][9];
+        bool ok_lo[2], ok_hi[2];
 #pragma unroll
-        for (int k = 0; k < kCubeIters; ++k) {
-            const int i = tid + k * kWaveThreads;
-            const int x = i % 9, y = (i / 9) % 9, z = i / 81;
-            const int o = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
-            const int32_t nbi = i < kCube ? c.nb[o] : -1;
-            present[k] = nbi >= 0;
-            q[k] = vox[(size_t)(present[k] ? nbi : bi) * kBlockVoxels +
-                       (((z & 7) * 8 + (y & 7)) * 8 + (x & 7))];
+        for (int h = 0; h < 2; ++h) {
+            const int r = tid + h * kWaveThreads;
+            const int y = r % 9, z = r / 9;
+            const int o = ((y >> 3) << 1) | ((z >> 3) << 2);
+            const int32_t nlo = c.nb[o & 7], nhi = c.nb[(o | 1) & 7];
+            ok_lo[h] = nlo >= 0;
+            ok_hi[h] = nhi >= 0;
+            const int row = ((z & 7) * 8 + (y & 7)) * 8;
+            const T* plo = vox + (size_t)(ok_lo[h] ? nlo : bi) * kBlockVoxels + row;
+            const T* phi = vox + (size_t)(ok_hi[h] ? nhi : bi) * kBlockVoxels + row;
+            if (h == 0 || r < kRows) {
+#pragma unroll
+                for (int x = 0; x < 8; ++x) q[h][x] = plo[x];
+                q[h][8] = phi[0];
+            }
         }
+        [[maybe_unused]] const uint32_t wfloor = f16_weight_floor(min_w);
 #pragma unroll
-        for (int k = 0; k < kCubeIters; ++k) {
-            const int i = tid + k * kWaveThreads;
-            if (i >= kCube) break;
-            if constexpr (STORE == kStoreF16) {
-                const float w = (float)(q[k] >> 16);
-                c.val[i] = present[k] && !(w < min_w || w <= 0.0f) ? (unsigned short)(q[k] & 0xffffu)
-                                                                    : (unsigned short)0x7e00u;
-            } else {
-                const float w = (float)q[k].y;
-                c.val[i] = present[k] && !(w < min_w || w <= 0.0f) ? (float)q[k].x
-                                                                    : __builtin_nanf("");
+        for (int h = 0; h < 2; ++h) {
+            const int r = tid + h * kWaveThreads;
+            if (h == 1 && r >= kRows) break;
+#pragma unroll
+            for (int x = 0; x < 9; ++x) {
+                const bool present = x < 8 ? ok_lo[h] : ok_hi[h];
+                if constexpr (STORE == kStoreF16) {
+                    c.val[r * 9 + x] = present && (q[h][x] >> 16) >= wfloor
+                                           ? (unsigned short)(q[h][x] & 0xffffu)
+                                           : (unsigned short)0x7e00u;
+                } else {
+                    const float w = (float)q[h][x].y;
+                    c.val[r * 9 + x] = present && !(w < min_w || w <= 0.0f) ? (float)q[h][x].x
+                                                                           : __builtin_nanf("");
+                }
             }
         }
     } else {
@@ -169,12 +190,12 @@ constexpr unsigned long long kTetBits = 0xC5B1D18D8BA3ull;
 // 1 one triangle, 2 a quad's two), the tetrahedra with a crossing in bits
 // 12-14 and the cube's vertices / 3 in bits 16-19. A tetrahedron with 1 or 3
 // corners below iso cuts one triangle, with 2 a quad.
-__device__ inline uint32_t classify_mask(uint32_t mask) {
+constexpr uint32_t classify_mask(uint32_t mask) {
     uint32_t e = 0, nt = 0, n3 = 0;
-#pragma unroll
     for (int t = 0; t < 6; ++t) {
-        const int pc = __popc(mask & (uint32_t)((kTetBits >> (8 * t)) & 0xffu));
-        const uint32_t f = pc == 2 ? 2u : (uint32_t)(pc & 1);
+        uint32_t m = mask & (uint32_t)((kTetBits >> (8 * t)) & 0xffu), pc = 0;
+        for (; m; m &= m - 1u) ++pc;
+        const uint32_t f = pc == 2u ? 2u : (pc & 1u);
         e |= f << (2 * t);
         nt += f != 0u;
         n3 += f;
@@ -182,10 +203,22 @@ __device__ inline uint32_t classify_mask(uint32_t mask) {
     return e | (nt << 12) | (n3 << 16);
 }
 
-// The classification of all 256 corner masks, in LDS; the workgroup fills it
-// once and reads it for every block it walks.
+struct MaskTable {
+    uint32_t e[256];
+};
+constexpr MaskTable make_mask_table() {
+    MaskTable t{};
+    for (uint32_t m = 0; m < 256; ++m) t.e[m] = classify_mask(m);
+    return t;
+}
+// Built by the compiler; every workgroup copies it into LDS (16 bytes per
+// lane) and reads it there for every block it walks.
+__constant__ MaskTable kMaskTable = make_mask_table();
+
 __device__ inline void build_table(uint32_t* table) {
-    for (int m = threadIdx.x; m < 256; m += blockDim.x) table[m] = classify_mask((uint32_t)m);
+    static_assert(sizeof(MaskTable) == kWaveThreads * 16, "one 16-byte piece per lane");
+    reinterpret_cast<uint4*>(table)[threadIdx.x] =
+        reinterpret_cast<const uint4*>(kMaskTable.e)[threadIdx.x];
     __syncthreads();
 }
 
@@ -197,20 +230,30 @@ __device__ inline void build_table(uint32_t* table) {
 template <int STORE>
 __device__ inline uint32_t classify_row(const Cube<STORE>& c, const uint32_t* table, int ly,
                                         int lz, float iso, uint32_t (&e)[8]) {
+    // FP16 corners compare as halves: a half x is below iso exactly when it
+    // is below iso rounded up to a half (no half lies in between)
+    [[maybe_unused]] const __half iso_h = __float2half_ru(iso);
+    auto below = [&](auto x) -> bool {
+        if constexpr (STORE == kStoreF16) return __hlt(__ushort_as_half(x), iso_h);
+        else return x < iso;
+    };
+    auto invalid = [&](auto x) -> bool {
+        if constexpr (STORE == kStoreF16) return __hisnan(__ushort_as_half(x));
+        else return __builtin_isnan(x);
+    };
     uint32_t prev_bits = 0, sum = 0;
     bool prev_ok = false;
 #pragma unroll
     for (int x = 0; x < 9; ++x) {
         // corners (x, ly + dy, lz + dz) at mask bit 2 dy + 4 dz: the cube's
         // corner ids with dx = 0
-        const float p = corner(c.val[cube_index(x, ly, lz)]);
-        const float q = corner(c.val[cube_index(x, ly + 1, lz)]);
-        const float r = corner(c.val[cube_index(x, ly, lz + 1)]);
-        const float s = corner(c.val[cube_index(x, ly + 1, lz + 1)]);
-        const bool ok = !(__builtin_isnan(p) || __builtin_isnan(q) || __builtin_isnan(r) ||
-                          __builtin_isnan(s));
-        const uint32_t bits = (p < iso ? 1u : 0u) | (q < iso ? 4u : 0u) | (r < iso ? 16u : 0u) |
-                              (s < iso ? 64u : 0u);
+        const auto p = c.val[cube_index(x, ly, lz)];
+        const auto q = c.val[cube_index(x, ly + 1, lz)];
+        const auto r = c.val[cube_index(x, ly, lz + 1)];
+        const auto s = c.val[cube_index(x, ly + 1, lz + 1)];
+        const bool ok = !(invalid(p) || invalid(q) || invalid(r) || invalid(s));
+        const uint32_t bits = (below(p) ? 1u : 0u) | (below(q) ? 4u : 0u) | (below(r) ? 16u : 0u) |
+                              (below(s) ? 64u : 0u);
         if (x > 0) {
             const uint32_t ent = (prev_ok && ok) ? table[prev_bits | (bits << 1)] : 0u;
             e[x - 1] = ent;
@@ -382,7 +425,7 @@ template <int STORE>
 __device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t first,
                              int32_t stride) {
     __shared__ Cube<STORE> c;
-    __shared__ uint32_t s_table[256];
+    __shared__ __attribute__((aligned(16))) uint32_t s_table[256];
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
@@ -505,7 +548,7 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan_batch(
 template <int STORE>
 __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first, int32_t stride) {
     __shared__ Cube<STORE> c;
-    __shared__ uint32_t s_table[256];
+    __shared__ __attribute__((aligned(16))) uint32_t s_table[256];
     __shared__ uint32_t s_tets[kTetCap];
     __shared__ Stage s_stage;
     if (*a.count_out < 0) return;  // overflow: write nothing
