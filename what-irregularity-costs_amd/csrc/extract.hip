@@ -24,6 +24,15 @@
 namespace osn_hip {
 namespace {
 
+// Device memory as the global address space. Pointers read from argument
+// arrays are otherwise generic, and generic (flat) accesses count against
+// the LDS wait counter too: every wait for an LDS read would then also wait
+// for the output stores in flight.
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T* global(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 constexpr int kWaveThreads = 64;     // count and emit: one wave per block, many per CU
 // emit's register bound: 5 waves per SIMD with FP16 corners (96 VGPRs, what
 // its LDS allows), 4 with FP32 (128), 2 with the colour planes of the
@@ -35,8 +44,8 @@ constexpr int kScanChunk = 1024;
 constexpr int kScanGrid = 256;      // scan workgroups (grid-stride over chunks)
 constexpr int kSerialChunks = 64;   // up to this many chunks one workgroup scans alone
 constexpr uint32_t kScanGridBatch = 32;  // the same per volume of a batch
-constexpr int kTetCap = 256;          // tetrahedra listed per pass over a block
-constexpr int kStageVerts = 288;      // vertices assembled in LDS per output group (measured best of 224-384)
+constexpr int kSlotCap = 1024;        // triangles listed per pass over a block
+constexpr int kStageVerts = 3 * kWaveThreads;  // one group: a triangle per lane
 
 // The 9x9x9 corner samples of one block; colour planes only where the
 // storage keeps colour (the reference's weighted sums). The FP16 storage's
@@ -61,9 +70,11 @@ struct Cube<kStoreSums> {
 
 // A voxel record as the packed storages keep it.
 template <int STORE> struct RawVox;
-template <> struct RawVox<kStoreF32> { using T = float2; };   // tsdf, weight
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <> struct RawVox<kStoreF32> { using T = f32x2; };    // tsdf, weight
 template <> struct RawVox<kStoreF16> { using T = uint32_t; }; // half tsdf | u16 weight << 16
-template <> struct RawVox<kStoreF64> { using T = double2; };  // tsdf, weight
+template <> struct RawVox<kStoreF64> { using T = f64x2; };    // tsdf, weight
 
 __device__ inline float corner(float x) { return x; }
 __device__ inline float corner(unsigned short h) { return __half2float(__ushort_as_half(h)); }
@@ -89,14 +100,14 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
         int32_t nbi = bi;
         if (LOOKUP) {
             if (tid != 0) {
-                const int32_t bx = v.block_coord[bi * 3 + 0] + (tid & 1);
-                const int32_t by = v.block_coord[bi * 3 + 1] + ((tid >> 1) & 1);
-                const int32_t bz = v.block_coord[bi * 3 + 2] + ((tid >> 2) & 1);
+                const int32_t bx = global(v.block_coord)[bi * 3 + 0] + (tid & 1);
+                const int32_t by = global(v.block_coord)[bi * 3 + 1] + ((tid >> 1) & 1);
+                const int32_t bz = global(v.block_coord)[bi * 3 + 2] + ((tid >> 2) & 1);
                 nbi = find_index(v, pack_coord(bx, by, bz));
             }
-            nb_out[(size_t)bi * 8 + tid] = nbi;
+            global(nb_out)[(size_t)bi * 8 + tid] = nbi;
         } else {
-            nbi = nb_out[(size_t)bi * 8 + tid];
+            nbi = global(nb_out)[(size_t)bi * 8 + tid];
         }
         c.nb[tid] = nbi;
     }
@@ -108,7 +119,7 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
         // all its loads before using any; a corner of an absent block reads
         // its own block (in bounds) and is then marked invalid.
         using T = typename RawVox<STORE>::T;
-        const T* vox = static_cast<const T*>(v.vox);
+        const auto vox = global(static_cast<const T*>(v.vox));
         constexpr int kRows = 81;
         T q[2][9];
         bool ok_lo[2], ok_hi[2];
@@ -121,8 +132,8 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
             ok_lo[h] = nlo >= 0;
             ok_hi[h] = nhi >= 0;
             const int row = ((z & 7) * 8 + (y & 7)) * 8;
-            const T* plo = vox + (size_t)(ok_lo[h] ? nlo : bi) * kBlockVoxels + row;
-            const T* phi = vox + (size_t)(ok_hi[h] ? nhi : bi) * kBlockVoxels + row;
+            const auto plo = vox + (size_t)(ok_lo[h] ? nlo : bi) * kBlockVoxels + row;
+            const auto phi = vox + (size_t)(ok_hi[h] ? nhi : bi) * kBlockVoxels + row;
             if (h == 0 || r < kRows) {
 #pragma unroll
                 for (int x = 0; x < 8; ++x) q[h][x] = plo[x];
@@ -303,7 +314,7 @@ __device__ inline void write_tri(const V6& a, const V6& b, const V6& c, Stage& s
         float* d = st.pos + (at + k) * 3;
         d[0] = vs[k]->p[0]; d[1] = vs[k]->p[1]; d[2] = vs[k]->p[2];
         if (rgb) {
-            uint8_t* q = rgb + (size_t)(at_rgb + k) * 3;
+            const auto q = global(rgb) + (size_t)(at_rgb + k) * 3;
             q[0] = (uint8_t)fminf(fmaxf(vs[k]->c[0], 0.0f), 255.0f);
             q[1] = (uint8_t)fminf(fmaxf(vs[k]->c[1], 0.0f), 255.0f);
             q[2] = (uint8_t)fminf(fmaxf(vs[k]->c[2], 0.0f), 255.0f);
@@ -328,15 +339,15 @@ __device__ inline int lex_key(int k) { return ((k & 1) << 2) | (k & 2) | ((k >> 
 constexpr unsigned long long kRolesLo = 0x93C9D8D2E4E1E400ull;  // codes 0-7
 constexpr unsigned long long kRolesHi = 0x00E4E14ED28D9C93ull;  // codes 8-15
 
-// The triangles of tetrahedron t of voxel (lx, ly, lz)'s cube, staged from
-// vertex index at, colours written from at_rgb. One path for triangles and
-// quads: three edge interpolations, a fourth and the second triangle for
-// quads only. Corner positions, values and colours are formed per corner id,
+// Triangle `half` (0, or 1 for a quad's second) of tetrahedron t of voxel
+// (lx, ly, lz)'s cube, staged from vertex index at, colours written from
+// at_rgb. Every lane takes the same path: three edge interpolations and one
+// triangle. Corner positions, values and colours are formed per corner id,
 // never held in an array indexed by a lane-varying value.
 template <int STORE>
-__device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int32_t by,
-                         int32_t bz, int lx, int ly, int lz, float iso, int t,
-                         Stage& st, int at, uint8_t* rgb, int64_t at_rgb) {
+__device__ void emit_slot(const Cube<STORE>& c, const VolDev& v, int32_t bx, int32_t by,
+                          int32_t bz, int lx, int ly, int lz, float iso, int t, bool half,
+                          Stage& st, int at, uint8_t* rgb, int64_t at_rgb) {
     constexpr bool kCol = STORE == kStoreSums;
     const int32_t vx = bx * 8 + lx, vy = by * 8 + ly, vz = bz * 8 + lz;
     // each axis' two corner coordinates (offset 0 and 1)
@@ -382,18 +393,16 @@ __device__ void emit_tet(const Cube<STORE>& c, const VolDev& v, int32_t bx, int3
     auto role = [&](int j) { return (int)((tc >> (4u * ((roles >> (2 * j)) & 3u))) & 15u); };
     const int a = role(0), b = role(1), cc = role(2), d = role(3);
     const bool quad = __popc((uint32_t)code) == 2;
+    // triangle (ab, ac, ad); quad halves (ac, ad, bd) and (ac, bd, bc)
+    const bool second = quad && half;
     const V6 e0 = interp(a, quad ? cc : b);
-    const V6 e1 = interp(a, quad ? d : cc);
-    const V6 e2 = interp(quad ? b : a, d);
+    const V6 e1 = second ? interp(b, d) : interp(a, quad ? d : cc);
+    const V6 e2 = quad ? (second ? interp(b, cc) : interp(b, d)) : interp(a, d);
     write_tri(e0, e1, e2, st, at, rgb, at_rgb);
-    if (quad) {
-        const V6 e3 = interp(b, cc);
-        write_tri(e0, e2, e3, st, at + 3, rgb, at_rgb + 3);
-    }
 }
 
 __device__ inline int32_t n_blocks_of(const VolDev& v, const ExtractArgs& a) {
-    return a.n_blocks >= 0 ? a.n_blocks : v.ctr->block_count;
+    return a.n_blocks >= 0 ? a.n_blocks : global(v.ctr)->block_count;
 }
 
 // Exclusive scan of one value per thread across the workgroup; returns the
@@ -435,7 +444,7 @@ __device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t firs
         uint32_t n3 = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e) & 0xffffu;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) n3 += __shfl_xor(n3, o);
-        if (lane == 0) a.block_vcount[b] = 3 * (int32_t)n3;
+        if (lane == 0) global(a.block_vcount)[b] = 3 * (int32_t)n3;
         __syncthreads();  // the cube is read before the next block replaces it
     }
 }
@@ -541,39 +550,37 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan_batch(
 }
 
 // Emit: one wave per block, lane = row (ly, lz) of 8 cubes. The block's
-// tetrahedra with a crossing are listed in LDS in output order (voxel, then
-// tetrahedron), each with its vertex offset, and emitted one per lane in
-// groups whose vertices fit the staging area; each group's vertices are one
-// contiguous run of the output, copied out 8 bytes per lane.
+// triangles are listed in LDS in output order (voxel, tetrahedron, half), so
+// triangle j's vertices are 3j.. of the block's run, and emitted one per lane
+// in groups of 64; each group's 192 vertices are staged and copied out as one
+// contiguous run.
 template <int STORE>
 __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first, int32_t stride) {
     __shared__ Cube<STORE> c;
     __shared__ __attribute__((aligned(16))) uint32_t s_table[256];
-    __shared__ uint32_t s_tets[kTetCap];
+    __shared__ uint16_t s_slots[kSlotCap];
     __shared__ Stage s_stage;
-    if (*a.count_out < 0) return;  // overflow: write nothing
+    if (*global(a.count_out) < 0) return;  // overflow: write nothing
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
     for (int32_t b = first; b < n; b += stride) {
-        if (a.block_vcount[b] == 0) continue;
+        if (global(a.block_vcount)[b] == 0) continue;
         load_cube<STORE, false>(v, b, a.min_weight, c, a.block_nb);
-        const int32_t bx = v.block_coord[b * 3 + 0];
-        const int32_t by = v.block_coord[b * 3 + 1];
-        const int32_t bz = v.block_coord[b * 3 + 2];
-        const int64_t base = (int64_t)a.chunk_sums[b / kScanChunk] + a.block_voff[b];
+        const int32_t bx = global(v.block_coord)[b * 3 + 0];
+        const int32_t by = global(v.block_coord)[b * 3 + 1];
+        const int32_t bz = global(v.block_coord)[b * 3 + 2];
+        const int64_t base = (int64_t)global(a.chunk_sums)[b / kScanChunk] + global(a.block_voff)[b];
         uint32_t e[8];
-        const uint32_t row = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e);
-        const uint32_t incl = wave_inclusive_scan(row);
-        const int n_tets = (int)((uint32_t)__builtin_amdgcn_readlane((int)incl, 63) >> 16);
-        const uint32_t before = incl - row;
-        // passes of up to kTetCap listed tetrahedra (one pass unless the
+        const uint32_t row = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e) & 0xffffu;
+        const uint32_t incl = wave_inclusive_scan(row);  // triangles = vertices / 3
+        const int n_tri = __builtin_amdgcn_readlane((int)incl, 63);
+        const int before = (int)(incl - row);
+        // passes of up to kSlotCap listed triangles (one pass unless the
         // block's surface is very dense)
-        for (int g0 = 0; g0 < n_tets; g0 += kTetCap) {
-            // entry: vertex offset in the block (bits 0-14, <= 512 x 36),
-            // quad (15), tetrahedron (16-18), voxel (19-27)
-            int k = (int)(before >> 16);
-            uint32_t vo = 3u * (before & 0xffffu);
+        for (int g0 = 0; g0 < n_tri; g0 += kSlotCap) {
+            // entry: voxel (bits 0-8), tetrahedron (9-11), half (12)
+            int k = before;
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
                 const uint32_t ent = e[x];
@@ -581,53 +588,45 @@ __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first
 #pragma nounroll
                     for (int t = 0; t < 6; ++t) {
                         const uint32_t f = (ent >> (2 * t)) & 3u;
-                        if (f) {
-                            if (k >= g0 && k < g0 + kTetCap)
-                                s_tets[k - g0] = vo | ((f - 1u) << 15) | ((uint32_t)t << 16) |
-                                                 ((uint32_t)(lane * 8 + x) << 19);
-                            ++k;
-                            vo += 3u * f;
-                        }
+                        const uint16_t item = (uint16_t)((lane * 8 + x) | (t << 9));
+                        if (f && (unsigned)(k - g0) < (unsigned)kSlotCap) s_slots[k - g0] = item;
+                        if (f == 2u && (unsigned)(k + 1 - g0) < (unsigned)kSlotCap)
+                            s_slots[k + 1 - g0] = (uint16_t)(item | (1u << 12));
+                        k += (int)f;
                     }
                 }
             }
             __syncthreads();
-            const int n_pass = min(kTetCap, n_tets - g0);
-            for (int g = 0; g < n_pass;) {
+            const int n_pass = min(kSlotCap, n_tri - g0);
+            for (int g = 0; g < n_pass; g += kWaveThreads) {
                 const int i = g + lane;
-                const uint32_t ent = i < n_pass ? s_tets[i] : 0u;
-                const int vo = (int)(ent & 0x7fffu);
-                const int v_lo = __builtin_amdgcn_readfirstlane(vo);  // lane 0: tetrahedron g
-                const int rel_end = vo + (((ent >> 15) & 1u) ? 6 : 3) - v_lo;
-                // the group: the longest run of listed tetrahedra whose
-                // vertices fit the staging area (at least one)
-                const unsigned long long fit = __ballot(i < n_pass && rel_end <= kStageVerts);
-                const int cnt = ~fit == 0ull ? 64 : __builtin_ctzll(~fit);
-                if (lane < cnt) {
-                    const int vi = (int)(ent >> 19), t = (int)((ent >> 16) & 7u);
-                    emit_tet(c, v, bx, by, bz, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, t,
-                             s_stage, vo - v_lo, a.rgb, base + vo);
+                const int cnt = min(kWaveThreads, n_pass - g);
+                const int64_t v_lo = base + 3 * (int64_t)(g0 + g);  // the group's first vertex
+                if (i < n_pass) {
+                    const uint32_t it = s_slots[i];
+                    const int vi = (int)(it & 511u), t = (int)((it >> 9) & 7u);
+                    emit_slot(c, v, bx, by, bz, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, t,
+                              ((it >> 12) & 1u) != 0u, s_stage, 3 * lane, a.rgb, v_lo + 3 * lane);
                 }
-                const int n_verts = __builtin_amdgcn_readlane(rel_end, cnt - 1);
+                const int n_verts = 3 * cnt;
                 __syncthreads();
-                float* out = a.posnor + (base + v_lo) * 6;
+                const auto out = global(a.posnor) + v_lo * 6;
                 if ((reinterpret_cast<uintptr_t>(out) & 7u) == 0) {
                     // one vertex per lane: three float2 stores, so each
                     // wave-wide store covers every third 8 bytes of one run
-                    float2* dst = reinterpret_cast<float2*>(out);
-                    for (int i = lane; i < n_verts; i += kWaveThreads) {
-                        const float* p = s_stage.pos + i * 3;
-                        const float* q = s_stage.nor + (i - i % 3);
-                        dst[i * 3 + 0] = make_float2(p[0], p[1]);
-                        dst[i * 3 + 1] = make_float2(p[2], q[0]);
-                        dst[i * 3 + 2] = make_float2(q[1], q[2]);
+                    const auto dst = (__attribute__((address_space(1))) f32x2*)out;
+                    for (int j = lane; j < n_verts; j += kWaveThreads) {
+                        const float* p = s_stage.pos + j * 3;
+                        const float* q = s_stage.nor + (j - j % 3);
+                        dst[j * 3 + 0] = f32x2{p[0], p[1]};
+                        dst[j * 3 + 1] = f32x2{p[2], q[0]};
+                        dst[j * 3 + 2] = f32x2{q[1], q[2]};
                     }
                 } else {  // a caller's buffer need only be float-aligned
                     for (int j = lane; j < n_verts * 6; j += kWaveThreads)
                         out[j] = staged(s_stage, j / 6, j % 6);
                 }
                 __syncthreads();
-                g += cnt;
             }
         }
     }
