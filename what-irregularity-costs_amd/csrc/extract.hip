@@ -394,10 +394,11 @@ __device__ void emit_slot(const Cube<STORE>& c, const VolDev& v, int32_t bx, int
     const int a = role(0), b = role(1), cc = role(2), d = role(3);
     const bool quad = __popc((uint32_t)code) == 2;
     // triangle (ab, ac, ad); quad halves (ac, ad, bd) and (ac, bd, bc)
+    // endpoints chosen first, so every lane runs the same three interpolations
     const bool second = quad && half;
     const V6 e0 = interp(a, quad ? cc : b);
-    const V6 e1 = second ? interp(b, d) : interp(a, quad ? d : cc);
-    const V6 e2 = quad ? (second ? interp(b, cc) : interp(b, d)) : interp(a, d);
+    const V6 e1 = interp(second ? b : a, second ? d : (quad ? d : cc));
+    const V6 e2 = interp(quad ? b : a, second ? cc : d);
     write_tri(e0, e1, e2, st, at, rgb, at_rgb);
 }
 
@@ -558,7 +559,7 @@ template <int STORE>
 __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first, int32_t stride) {
     __shared__ Cube<STORE> c;
     __shared__ __attribute__((aligned(16))) uint32_t s_table[256];
-    __shared__ uint16_t s_slots[kSlotCap];
+    __shared__ uint16_t s_slots[kSlotCap + kWaveThreads];  // + one dummy entry per lane
     __shared__ Stage s_stage;
     if (*global(a.count_out) < 0) return;  // overflow: write nothing
     build_table(s_table);
@@ -581,6 +582,22 @@ __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first
         for (int g0 = 0; g0 < n_tri; g0 += kSlotCap) {
             // entry: voxel (bits 0-8), tetrahedron (9-11), half (12)
             int k = before;
+            if (n_tri <= kSlotCap) {
+                // one pass: no range checks, no branches; an item with
+                // nothing to list writes this lane's dummy entry instead
+#pragma unroll
+                for (int x = 0; x < 8; ++x) {
+                    const uint32_t ent = e[x];
+#pragma nounroll
+                    for (int t = 0; t < 6; ++t) {
+                        const uint32_t f = (ent >> (2 * t)) & 3u;
+                        const uint16_t item = (uint16_t)((lane * 8 + x) | (t << 9));
+                        s_slots[f ? k : kSlotCap + lane] = item;
+                        s_slots[f == 2u ? k + 1 : kSlotCap + lane] = (uint16_t)(item | (1u << 12));
+                        k += (int)f;
+                    }
+                }
+            } else
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
                 const uint32_t ent = e[x];
