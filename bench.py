@@ -229,6 +229,37 @@ def run_chain(H, cfg, storage, scenes, warmup, steps, batched):
     return dt, out
 
 
+def run_chain_instrumented(H, cfg, storage, scenes, warmup, steps):
+    """The batched chain through the eager path with per-stage HIP-event times
+    (allocate, assign, update, extract = count + scan + emit of every scene),
+    ms per step over `steps` steps after `warmup` untimed ones, and the
+    vertices of the last step's meshes (the meshes grow a little per step:
+    keep `steps` small for a per-step byte figure)."""
+    import torch
+    n = len(scenes)
+    bufs = [torch.empty(CHAIN_VERTS * 6, dtype=torch.float32, device="cuda") for _ in range(n)]
+    counts = torch.zeros(n, dtype=torch.int32, device="cuda")
+    b, vols = make_batch(H, cfg, storage, scenes)
+    if b.set_extract(bufs, [CHAIN_VERTS] * n, counts, 0.5, 0.0, every=1):
+        raise RuntimeError("set_extract failed")
+    if b.run(warmup):
+        raise RuntimeError("batch run failed")
+    b.synchronize()
+    b.read_timing()
+    b.set_timing(True)
+    if b.run(steps):
+        raise RuntimeError("batch run failed")
+    b.synchronize()
+    ms, calls = b.read_timing()
+    out = counts.cpu().tolist()
+    b.close()
+    for v in vols:
+        v.close()
+    del bufs
+    torch.cuda.empty_cache()
+    return ms / np.maximum(calls, 1), out
+
+
 PRECISION_MESH_VERTS = 40 << 20
 
 
@@ -331,6 +362,9 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="only the batched fp16 run and one 256 MiB device copy (the copy "
                          "calibrates HBM byte counters), for rocprofv3 passes")
+    ap.add_argument("--profile-chain", action="store_true",
+                    help="only the batched fp16 chain with extraction every frame (graph, "
+                         "then the instrumented eager pass) and the calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -393,6 +427,23 @@ def main():
             f"{alg_all.mean() / 1e6:.1f} MB")
         return
 
+    if args.profile_chain:
+        src = torch.empty(1 << 26, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        dst.copy_(src)  # 256 MiB read + 256 MiB written
+        torch.cuda.synchronize()
+        dt, verts = run_chain(H, cfg, H.STORE_F16, scenes, args.warmup, args.steps, True)
+        log(f"chain batched fp16 {frames_timed / dt:.0f} frames/s")
+        per, verts = run_chain_instrumented(H, cfg, H.STORE_F16, scenes,
+                                            args.warmup + args.steps - 3, 3)
+        out_b = sum(verts) * 24
+        log(f"stage ms per step: allocate {per[0]:.5f} assign {per[1]:.5f} update {per[2]:.5f} "
+            f"extract {per[3]:.5f} (last 3 steps); last step {sum(verts)} vertices, "
+            f"{out_b / 1e6:.1f} MB of mesh -> {out_b / (per[3] * 1e-3) / 1e9:.0f} GB/s over "
+            f"the extract stage")
+        return
+
     log("batched fp16 (timed)")
     dt_b16, blocks = run_batched(H, cfg, H.STORE_F16, scenes, args.warmup, args.steps,
                                  graph=not args.no_graph)
@@ -420,6 +471,13 @@ def main():
         "last_mesh_vertices_fp32": verts_p,
     }
     chain["ratio"] = round(chain["batched_fp16_frames_per_s"] / chain["plain_fp32_frames_per_s"], 3)
+    per, verts_i = run_chain_instrumented(H, cfg, H.STORE_F16, scenes,
+                                          args.warmup + chain_steps - 3, 3)
+    chain["batched_fp16_stage_ms_per_step"] = {
+        "allocate": round(float(per[0]), 5), "assign": round(float(per[1]), 5),
+        "update": round(float(per[2]), 5), "extract": round(float(per[3]), 5)}
+    chain["extract_mesh_bytes_per_step"] = sum(verts_i) * 24
+    chain["extract_write_gb_per_s"] = round(sum(verts_i) * 24 / (float(per[3]) * 1e-3) / 1e9, 1)
     log(f"chain batched fp16 {chain['batched_fp16_frames_per_s']:.0f} plain fp32 "
         f"{chain['plain_fp32_frames_per_s']:.0f}")
 
