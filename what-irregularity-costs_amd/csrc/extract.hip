@@ -34,17 +34,17 @@ __device__ inline __attribute__((address_space(1))) T* global(T* p) {
 }
 
 constexpr int kWaveThreads = 64;     // count and emit: one wave per block, many per CU
-// emit's register bound: 5 waves per SIMD with FP16 corners (96 VGPRs, what
+// emit's register bound: 6 waves per SIMD with FP16 corners (80 VGPRs, what
 // its LDS allows), 4 with FP32 (128), 2 with the colour planes of the
 // weighted sums
 template <int STORE>
-constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : (STORE == kStoreF16 ? 5 : 4); }
+constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : (STORE == kStoreF16 ? 6 : 4); }
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanChunk = 1024;
 constexpr int kScanGrid = 256;      // scan workgroups (grid-stride over chunks)
 constexpr int kSerialChunks = 64;   // up to this many chunks one workgroup scans alone
 constexpr uint32_t kScanGridBatch = 32;  // the same per volume of a batch
-constexpr int kSlotCap = 1024;        // triangles listed per pass over a block
+constexpr int kSlotCap = 768;         // triangles listed per pass over a block
 constexpr int kStageVerts = 3 * kWaveThreads;  // one group: a triangle per lane
 
 // The 9x9x9 corner samples of one block; colour planes only where the
@@ -197,49 +197,46 @@ __device__ inline uint32_t tet_corners(int t) {
 constexpr unsigned long long kTetBits = 0xC5B1D18D8BA3ull;
 
 // Classification of a cube from its corner mask (bit k set when corner k is
-// below iso): per tetrahedron t its vertices / 3 in bits 2t..2t+1 (0 none,
-// 1 one triangle, 2 a quad's two), the tetrahedra with a crossing in bits
-// 12-14 and the cube's vertices / 3 in bits 16-19. A tetrahedron with 1 or 3
-// corners below iso cuts one triangle, with 2 a quad.
-constexpr uint32_t classify_mask(uint32_t mask) {
-    uint32_t e = 0, nt = 0, n3 = 0;
+// below iso): per tetrahedron t its triangles in bits 2t..2t+1 (0 none, 1 one,
+// 2 a quad's two) and the cube's triangles (vertices / 3) in bits 12-15. A
+// tetrahedron with 1 or 3 corners below iso cuts one triangle, with 2 a quad.
+constexpr uint16_t classify_mask(uint32_t mask) {
+    uint32_t e = 0, n3 = 0;
     for (int t = 0; t < 6; ++t) {
         uint32_t m = mask & (uint32_t)((kTetBits >> (8 * t)) & 0xffu), pc = 0;
         for (; m; m &= m - 1u) ++pc;
         const uint32_t f = pc == 2u ? 2u : (pc & 1u);
         e |= f << (2 * t);
-        nt += f != 0u;
         n3 += f;
     }
-    return e | (nt << 12) | (n3 << 16);
+    return (uint16_t)(e | (n3 << 12));
 }
 
 struct MaskTable {
-    uint32_t e[256];
+    uint16_t e[256];
 };
 constexpr MaskTable make_mask_table() {
     MaskTable t{};
     for (uint32_t m = 0; m < 256; ++m) t.e[m] = classify_mask(m);
     return t;
 }
-// Built by the compiler; every workgroup copies it into LDS (16 bytes per
+// Built by the compiler; every workgroup copies it into LDS (8 bytes per
 // lane) and reads it there for every block it walks.
 __constant__ MaskTable kMaskTable = make_mask_table();
 
-__device__ inline void build_table(uint32_t* table) {
-    static_assert(sizeof(MaskTable) == kWaveThreads * 16, "one 16-byte piece per lane");
-    reinterpret_cast<uint4*>(table)[threadIdx.x] =
-        reinterpret_cast<const uint4*>(kMaskTable.e)[threadIdx.x];
+__device__ inline void build_table(uint16_t* table) {
+    static_assert(sizeof(MaskTable) == kWaveThreads * 8, "one 8-byte piece per lane");
+    reinterpret_cast<uint2*>(table)[threadIdx.x] =
+        reinterpret_cast<const uint2*>(kMaskTable.e)[threadIdx.x];
     __syncthreads();
 }
 
 // The row of cubes (0..7, ly, lz) of the staged block: per cube its
 // classification in e[x] (0 when a corner is not a valid sample). Corner
 // columns x and x + 1 are shared by neighbouring cubes, so the row reads 36
-// corners, not 64. Returns the row's tetrahedra with a crossing << 16 |
-// vertices / 3.
+// corners, not 64. Returns the row's triangles (vertices / 3).
 template <int STORE>
-__device__ inline uint32_t classify_row(const Cube<STORE>& c, const uint32_t* table, int ly,
+__device__ inline uint32_t classify_row(const Cube<STORE>& c, const uint16_t* table, int ly,
                                         int lz, float iso, uint32_t (&e)[8]) {
     // FP16 corners compare as halves: a half x is below iso exactly when it
     // is below iso rounded up to a half (no half lies in between)
@@ -268,7 +265,7 @@ __device__ inline uint32_t classify_row(const Cube<STORE>& c, const uint32_t* ta
         if (x > 0) {
             const uint32_t ent = (prev_ok && ok) ? table[prev_bits | (bits << 1)] : 0u;
             e[x - 1] = ent;
-            sum += (ent >> 16) | (((ent >> 12) & 7u) << 16);
+            sum += ent >> 12;
         }
         prev_bits = bits;
         prev_ok = ok;
@@ -435,14 +432,14 @@ template <int STORE>
 __device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t first,
                              int32_t stride) {
     __shared__ Cube<STORE> c;
-    __shared__ __attribute__((aligned(16))) uint32_t s_table[256];
+    __shared__ __attribute__((aligned(16))) uint16_t s_table[256];
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
     for (int32_t b = first; b < n; b += stride) {
         load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb);
         uint32_t e[8];
-        uint32_t n3 = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e) & 0xffffu;
+        uint32_t n3 = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) n3 += __shfl_xor(n3, o);
         if (lane == 0) global(a.block_vcount)[b] = 3 * (int32_t)n3;
@@ -558,7 +555,7 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan_batch(
 template <int STORE>
 __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first, int32_t stride) {
     __shared__ Cube<STORE> c;
-    __shared__ __attribute__((aligned(16))) uint32_t s_table[256];
+    __shared__ __attribute__((aligned(16))) uint16_t s_table[256];
     __shared__ uint16_t s_slots[kSlotCap + kWaveThreads];  // + one dummy entry per lane
     __shared__ Stage s_stage;
     if (*global(a.count_out) < 0) return;  // overflow: write nothing
@@ -573,8 +570,8 @@ __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first
         const int32_t bz = global(v.block_coord)[b * 3 + 2];
         const int64_t base = (int64_t)global(a.chunk_sums)[b / kScanChunk] + global(a.block_voff)[b];
         uint32_t e[8];
-        const uint32_t row = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e) & 0xffffu;
-        const uint32_t incl = wave_inclusive_scan(row);  // triangles = vertices / 3
+        const uint32_t row = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e);
+        const uint32_t incl = wave_inclusive_scan(row);
         const int n_tri = __builtin_amdgcn_readlane((int)incl, 63);
         const int before = (int)(incl - row);
         // passes of up to kSlotCap listed triangles (one pass unless the
