@@ -351,7 +351,12 @@ __device__ void emit_slot(const Cube<STORE>& c, const VolDev& v, int32_t bx, int
     const float x0 = ((float)vx + 0.5f) * v.voxel, x1 = ((float)(vx + 1) + 0.5f) * v.voxel;
     const float y0 = ((float)vy + 0.5f) * v.voxel, y1 = ((float)(vy + 1) + 0.5f) * v.voxel;
     const float z0 = ((float)vz + 0.5f) * v.voxel, z1 = ((float)(vz + 1) + 0.5f) * v.voxel;
-    auto ci = [&](int k) { return cube_index(lx + (k & 1), ly + ((k >> 1) & 1), lz + ((k >> 2) & 1)); };
+    // corner k at cube index base + dx + 9 dy + 81 dz (24-bit multiply-adds)
+    const int cbase = cube_index(lx, ly, lz);
+    auto ci = [&](int k) {
+        return cbase + (k & 1) + (int)__umul24((unsigned)((k >> 1) & 1), 9u) +
+               (int)__umul24((unsigned)(k >> 2), 81u);
+    };
     auto val = [&](int k) { return corner(c.val[ci(k)]); };
     // the point of edge (ka, kb) where the mean tsdf crosses iso, endpoints in
     // position order (cpp/src/marching_tets.cpp:37-80)
