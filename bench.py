@@ -234,7 +234,7 @@ def run_chain_instrumented(H, cfg, storage, scenes, warmup, steps):
     (allocate, assign, update, extract = count + scan + emit of every scene),
     ms per step over `steps` steps after `warmup` untimed ones, and the
     vertices of the last step's meshes (the meshes grow a little per step:
-    keep `steps` small for a per-step byte figure)."""
+    keep `steps` small for a per-step byte figure) and the blocks allocated."""
     import torch
     n = len(scenes)
     bufs = [torch.empty(CHAIN_VERTS * 6, dtype=torch.float32, device="cuda") for _ in range(n)]
@@ -252,12 +252,13 @@ def run_chain_instrumented(H, cfg, storage, scenes, warmup, steps):
     b.synchronize()
     ms, calls = b.read_timing()
     out = counts.cpu().tolist()
+    blocks = sum(v.block_count for v in vols)  # waits for the batch
     b.close()
     for v in vols:
         v.close()
     del bufs
     torch.cuda.empty_cache()
-    return ms / np.maximum(calls, 1), out
+    return ms / np.maximum(calls, 1), out, blocks
 
 
 PRECISION_MESH_VERTS = 40 << 20
@@ -435,8 +436,8 @@ def main():
         torch.cuda.synchronize()
         dt, verts = run_chain(H, cfg, H.STORE_F16, scenes, args.warmup, args.steps, True)
         log(f"chain batched fp16 {frames_timed / dt:.0f} frames/s")
-        per, verts = run_chain_instrumented(H, cfg, H.STORE_F16, scenes,
-                                            args.warmup + args.steps - 3, 3)
+        per, verts, _ = run_chain_instrumented(H, cfg, H.STORE_F16, scenes,
+                                               args.warmup + args.steps - 3, 3)
         out_b = sum(verts) * 24
         log(f"stage ms per step: allocate {per[0]:.5f} assign {per[1]:.5f} update {per[2]:.5f} "
             f"extract {per[3]:.5f} (last 3 steps); last step {sum(verts)} vertices, "
@@ -471,13 +472,22 @@ def main():
         "last_mesh_vertices_fp32": verts_p,
     }
     chain["ratio"] = round(chain["batched_fp16_frames_per_s"] / chain["plain_fp32_frames_per_s"], 3)
-    per, verts_i = run_chain_instrumented(H, cfg, H.STORE_F16, scenes,
-                                          args.warmup + chain_steps - 3, 3)
+    per, verts_i, blocks_i = run_chain_instrumented(H, cfg, H.STORE_F16, scenes,
+                                                    args.warmup + chain_steps - 3, 3)
     chain["batched_fp16_stage_ms_per_step"] = {
         "allocate": round(float(per[0]), 5), "assign": round(float(per[1]), 5),
         "update": round(float(per[2]), 5), "extract": round(float(per[3]), 5)}
-    chain["extract_mesh_bytes_per_step"] = sum(verts_i) * 24
-    chain["extract_write_gb_per_s"] = round(sum(verts_i) * 24 / (float(per[3]) * 1e-3) / 1e9, 1)
+    # extract's algorithmic bytes: every allocated block's voxels read once
+    # (FP16 + u16: 4 B) and 24 B per vertex written; count and emit in fact
+    # read the blocks twice (emit only the non-empty ones), with neighbour faces
+    mesh_b = sum(verts_i) * 24
+    alg_b = blocks_i * 512 * 4 + mesh_b
+    chain["extract_mesh_bytes_per_step"] = mesh_b
+    chain["extract_roofline"] = {
+        "bound": "hbm", "blocks": blocks_i, "algorithmic_bytes": alg_b,
+        "achieved": round(alg_b / (float(per[3]) * 1e-3) / 1e9, 1), "peak": 8000.0,
+        "unit": "GB/s", "frac": round(alg_b / (float(per[3]) * 1e-3) / 8e12, 4),
+        "stage": "count + scan + emit of all 16 scenes, events around the launches"}
     log(f"chain batched fp16 {chain['batched_fp16_frames_per_s']:.0f} plain fp32 "
         f"{chain['plain_fp32_frames_per_s']:.0f}")
 
