@@ -230,6 +230,41 @@ def test_device_extract_matches_host_extract(H, S):
     assert np.array_equal(ov.extract(1.0, 0.0), host)
 
 
+@pytest.mark.parametrize("storage", ["STORE_F16", "STORE_F32"])
+def test_dense_blocks_extract_bit_identical(H, storage):
+    """Independent random depths per pixel, about one pixel per voxel: nearly
+    every cube of the band crosses, so blocks hold more triangles than one
+    listing pass takes (768). Device extraction into host buffers, and into a
+    device buffer one float past an 8-byte boundary, against the oracle
+    extractor over the same voxel state (marching_tets.cpp:84-243)."""
+    W, Hh = 128, 96
+    fr = H.make_frame(W, Hh, 128.0, 128.0, 64.0, 48.0, np.eye(3), np.zeros(3))
+    rng = np.random.default_rng(17)
+    depth = torch.from_numpy(rng.uniform(0.95, 1.05, W * Hh).astype(np.float32)).cuda()
+    v = H.HipVolume(0.01, 1 << 13, 0.04, 32.0, storage=getattr(H, storage))
+    assert v.integrate_depth(depth, fr) == 0
+    host = v.extract(0.5, 0.0, cap=4 << 20)
+    assert host is not None
+    n = len(host)
+    # the densest blocks exceed one listing pass
+    cell = np.floor(host[:, :3].astype(np.float64) / 0.01 - 0.5).astype(np.int64)
+    _, per_block = np.unique(np.floor_divide(cell, 8), axis=0, return_counts=True)
+    print("vertices", n, "blocks", v.block_count, "max triangles per block", per_block.max() // 3)
+    assert per_block.max() > 3 * 768
+    keys = v.keys()
+    t, w = v.read_voxels()
+    ov = B.Volume(0.01, len(keys) + 8, 0.04, 0.0)
+    for i, k in enumerate(keys):
+        ov.load_block(int(k), t[i], w[i], 1)
+    assert np.array_equal(ov.extract(0.5, 0.0, cap=4 << 20), host)
+    buf = torch.empty(n * 6 + 8, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    assert v.extract_device(buf[1:], None, n, 0.5, 0.0, cnt) == 0
+    v.synchronize()
+    assert int(cnt.item()) == n
+    assert np.array_equal(buf[1:1 + n * 6].cpu().numpy().reshape(n, 6), host)
+
+
 def test_storage_errors(H, S):
     v = H.HipVolume(0.02, 1024, -1.0, 32.0, storage=H.STORE_F16)
     pts = torch.zeros(30, dtype=torch.float32, device="cuda")
