@@ -102,9 +102,9 @@ __device__ inline void block_forms(const Params& k, int32_t bx, int32_t by, int3
 // [kZMin, kZMax] (a voxel outside that range is rejected anyway), form the
 // running products, divide once (correctly rounded), and peel the reciprocals
 // off with the partial products. Six multiplies and one division instead of
-// four divisions; a few ulp from 1 / z, every step rounded on its own.
-__device__ inline void run_reciprocals(const float Z[4], float iz[4]) {
-    float z[4];
+// four divisions; a few ulp from 1 / z, every step rounded on its own. z gets
+// the clamped depths: Z is in range exactly when z == Z (NaN included).
+__device__ inline void run_reciprocals(const float Z[4], float iz[4], float z[4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) z[j] = fminf(fmaxf(Z[j], kZMin), kZMax);
     const float p1 = z[0] * z[1];
@@ -219,7 +219,7 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
     bool inb[8];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-        float A[4], B[4], Z[4], iz[4];
+        float A[4], B[4], Z[4], iz[4], zc[4];
         A[0] = f0[0] + L.off[r][0];
         B[0] = f0[1] + L.off[r][1];
         Z[0] = f0[2] + L.off[r][2];
@@ -229,12 +229,12 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
             B[j] = B[j - 1] + k.g[1][0];
             Z[j] = Z[j - 1] + k.g[2][0];
         }
-        run_reciprocals(Z, iz);
+        run_reciprocals(Z, iz, zc);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const float uf = A[j] * iz[j];
             const float vf = B[j] * iz[j];
-            const bool in = (Z[j] >= kZMin) & (Z[j] <= kZMax) & (uf >= 0.0f) & (vf >= 0.0f) &
+            const bool in = (zc[j] == Z[j]) & (uf >= 0.0f) & (vf >= 0.0f) &
                             (uf < (float)k.W) & (vf < (float)k.H);
             // uf, vf >= 0: truncation is floor; the depth is read from the
             // 8x4-tiled copy of the image
