@@ -141,7 +141,7 @@ hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipS
 // Count + scan (emit false) or emit for every volume of a batch, one launch
 // each: blockIdx.y is the volume, grid_per_scene one-wave workgroups walk its
 // blocks (fewer when the pools are smaller).
-constexpr int32_t kExtractGridPerScene = 8192;
+constexpr int32_t kExtractGridPerScene = 16384;
 hipError_t launch_extract_batch(const VolDev* vols, const ExtractArgs* args, int32_t n_scenes,
                                 int32_t storage, uint32_t max_pool_capacity,
                                 int32_t grid_per_scene, bool emit, hipStream_t s);
