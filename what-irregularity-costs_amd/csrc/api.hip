@@ -218,7 +218,12 @@ void OsnTsdfHipVolume::stage_end(int stage, hipEvent_t a) {
 
 bool OsnTsdfHipVolume::grow_extract(int32_t blocks) {
     if (blocks <= ex_blocks_cap && ex_vcount) return true;
-    int32_t want = blocks > 1024 ? blocks : 1024;
+    // geometric growth (to the pool at most): a growing volume regrows, and
+    // loses its cached neighbour indices, a few times rather than every call
+    int64_t want = blocks > 1024 ? blocks : 1024;
+    if (want < 2 * (int64_t)ex_blocks_cap) want = 2 * (int64_t)ex_blocks_cap;
+    if (want > cfg.pool_capacity_blocks && blocks <= cfg.pool_capacity_blocks)
+        want = cfg.pool_capacity_blocks;
     if (ex_vcount) { hipFree(ex_vcount); hipFree(ex_voff); hipFree(ex_chunks); hipFree(ex_nb); }
     ex_vcount = ex_voff = ex_chunks = ex_nb = nullptr;
     HIP_OK(hipMalloc(&ex_nb, (size_t)want * 8 * sizeof(int32_t)));
@@ -229,7 +234,11 @@ bool OsnTsdfHipVolume::grow_extract(int32_t blocks) {
         HIP_OK(hipMalloc(&ex_flag_count, 2 * sizeof(int32_t)));
         HIP_OK(hipMemsetAsync(ex_flag_count, 0, 2 * sizeof(int32_t), stream));
     }
-    ex_blocks_cap = want;
+    ex_blocks_cap = (int32_t)want;
+    // the neighbour indices cached in the old scratch are gone; complete
+    // before returning, since a batch's launches run on another stream
+    HIP_OK(hipMemsetAsync(&d.ctr->nb_blocks, 0, sizeof(int32_t), stream));
+    HIP_OK(hipStreamSynchronize(stream));
     return true;
 }
 

@@ -73,7 +73,8 @@ struct Counters {
     unsigned long long wave_steps;   // sum over wave batches of 64 * deepest lane
     unsigned long long lookups;      // distinct-key probes issued to the table
     int32_t mesh_vertices;           // last extract's vertex count, -1 on overflow
-    int32_t pad;
+    int32_t nb_blocks;               // blocks 0..nb_blocks-1 have their neighbour indices
+                                     // in the extraction scratch (-1 entries are looked up again)
     unsigned long long probe_hist[kProbeHistBins];
     ListCounters lists[2];
 };

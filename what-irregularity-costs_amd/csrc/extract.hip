@@ -91,15 +91,20 @@ __device__ inline uint32_t f16_weight_floor(float min_w) {
 
 // Stage block bi's corner samples. The count pass (LOOKUP) finds the 7
 // neighbour blocks in the table and records their indices in nb_out[bi * 8];
-// the emit pass reads them back from there.
+// the emit pass reads them back from there. Pool indices never move until a
+// reset, so the count pass reuses the indices it found in earlier calls for
+// blocks below `cached` and looks up only neighbours that were absent then.
 template <int STORE, bool LOOKUP>
 __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>& c,
-                          int32_t* nb_out) {
+                          int32_t* nb_out, int32_t cached = 0) {
     const int tid = threadIdx.x;
     if (tid < 8) {
         int32_t nbi = bi;
         if (LOOKUP) {
-            if (tid != 0) {
+            const int32_t known = bi < cached ? global(nb_out)[(size_t)bi * 8 + tid] : -1;
+            if (known >= 0) {
+                nbi = known;
+            } else if (tid != 0) {
                 const int32_t bx = global(v.block_coord)[bi * 3 + 0] + (tid & 1);
                 const int32_t by = global(v.block_coord)[bi * 3 + 1] + ((tid >> 1) & 1);
                 const int32_t bz = global(v.block_coord)[bi * 3 + 2] + ((tid >> 2) & 1);
@@ -441,8 +446,9 @@ __device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t firs
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
+    const int32_t cached = global(v.ctr)->nb_blocks;
     for (int32_t b = first; b < n; b += stride) {
-        load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb);
+        load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb, cached);
         uint32_t e[8];
         uint32_t n3 = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e);
 #pragma unroll
@@ -474,6 +480,8 @@ __device__ void scan_chunks(const VolDev& v, const ExtractArgs& a) {
     __shared__ int s_last;
     const int32_t n = n_blocks_of(v, a);
     const int n_chunks = (n + kScanChunk - 1) / kScanChunk;
+    // the count pass that just ran holds every block's neighbour indices
+    if (blockIdx.x == 0 && threadIdx.x == 0) global(v.ctr)->nb_blocks = n;
     if (n == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             *a.count_out = 0;
