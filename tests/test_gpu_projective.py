@@ -265,6 +265,28 @@ def test_dense_blocks_extract_bit_identical(H, storage):
     assert np.array_equal(buf[1:1 + n * 6].cpu().numpy().reshape(n, 6), host)
 
 
+def test_extract_after_every_frame_and_reset(H, S):
+    """Extraction after every frame reuses the neighbour indices the earlier
+    calls found; blocks allocated later next to old ones and a reset must not
+    leave stale indices. The last mesh of each round against the oracle
+    extractor over the same voxel state."""
+    cfg = dict(S.SMALL)
+    _, fr = frames_for(H, S, cfg, 5, 6)
+    v = H.HipVolume(cfg["voxel"], 1 << 14, cfg["trunc"], 32.0, storage=H.STORE_F16)
+    for _ in range(2):
+        for f, d, _, _ in fr:
+            assert v.integrate_depth(d, f) == 0
+            assert v.extract(0.5, 0.0, cap=4 << 20) is not None
+        host = v.extract(0.5, 0.0, cap=4 << 20)
+        keys = v.keys()
+        t, w = v.read_voxels()
+        ov = B.Volume(cfg["voxel"], len(keys) + 8, cfg["trunc"], 0.0)
+        for i, k in enumerate(keys):
+            ov.load_block(int(k), t[i], w[i], 1)
+        assert len(host) > 0 and np.array_equal(ov.extract(0.5, 0.0, cap=4 << 20), host)
+        assert v.reset() == 0
+
+
 def test_storage_errors(H, S):
     v = H.HipVolume(0.02, 1024, -1.0, 32.0, storage=H.STORE_F16)
     pts = torch.zeros(30, dtype=torch.float32, device="cuda")
