@@ -839,7 +839,8 @@ __device__ void lds_sort(AssignLds& s, int cnt) {
 // Number of A entries among the first k outputs of merge(A, B), keys
 // distinct, by one wave: each round tests 64 evenly spaced candidates of the
 // monotone predicate A[i] < B[k - i - 1] and keeps the gap where it turns.
-__device__ int corank_wave(const TouchEntry* A, int la, const TouchEntry* B, int lb, int k) {
+__device__ int corank_wave(const TouchEntry OSN_GLOBAL* A, int la, const TouchEntry OSN_GLOBAL* B,
+                           int lb, int k) {
     int lo = max(0, k - lb), hi = min(k, la);
     const int lane = __lane_id();
     while (lo < hi) {
@@ -876,19 +877,23 @@ struct AssignCall {
 __device__ inline void emit_sorted(const VolDev& v, const AssignCall& a, bool top, TouchEntry* dst,
                                    int p, int64_t key, uint32_t slot, unsigned long long& drops) {
     if (!top) {
-        dst[p] = TouchEntry{key, slot, kIdxPending};
+        TouchEntry OSN_GLOBAL* d = glob(dst) + p;
+        d->key = key;
+        d->slot = slot;
+        d->bi = kIdxPending;
         return;
     }
     const int64_t idx = (int64_t)a.bc0 + p;
     if (idx < v.pool_capacity) {
         int32_t x, y, z;
         unpack_key(key, x, y, z);
-        v.block_coord[idx * 3 + 0] = x;
-        v.block_coord[idx * 3 + 1] = y;
-        v.block_coord[idx * 3 + 2] = z;
-        v.table[slot].block_idx = (int32_t)idx;
+        int32_t OSN_GLOBAL* bc = glob(v.block_coord) + idx * 3;
+        bc[0] = x;
+        bc[1] = y;
+        bc[2] = z;
+        glob(v.table)[slot].block_idx = (int32_t)idx;
     } else {
-        v.table[slot].block_idx = kIdxDropped;
+        glob(v.table)[slot].block_idx = kIdxDropped;
         drops += 1;
     }
 }
@@ -954,9 +959,9 @@ __device__ bool assign_item(const VolDev& v, const AssignCall& a, AssignLds& s, 
     if (level == 0) {
         const int cnt = o1 - o0;
         for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-            const TouchEntry e = v.new_keys[o0 + i];
-            s.key[i] = e.key;
-            s.slot[i] = e.slot;
+            const TouchEntry OSN_GLOBAL* e = glob(v.new_keys) + o0 + i;
+            s.key[i] = e->key;
+            s.slot[i] = e->slot;
         }
         __syncthreads();
         lds_sort(s, cnt);
@@ -976,12 +981,12 @@ __device__ bool assign_item(const VolDev& v, const AssignCall& a, AssignLds& s, 
         const int m = o1 - o0;
         if (a_hi >= b_hi) {  // a lone run: copied (or assigned) as it is
             for (int i = threadIdx.x; i < m; i += blockDim.x) {
-                const TouchEntry e = src[o0 + i];
-                emit_sorted(v, a, top, dst, o0 + i, e.key, e.slot, drops);
+                const TouchEntry OSN_GLOBAL* e = glob(src) + o0 + i;
+                emit_sorted(v, a, top, dst, o0 + i, e->key, e->slot, drops);
             }
         } else {
-            const TouchEntry* A = src + a_lo;
-            const TouchEntry* B = src + a_hi;
+            const TouchEntry OSN_GLOBAL* A = glob(src) + a_lo;
+            const TouchEntry OSN_GLOBAL* B = glob(src) + a_hi;
             const int la = a_hi - a_lo, lb = b_hi - a_hi;
             const int wave = threadIdx.x >> 6;
             if (wave < 2) {
@@ -993,9 +998,9 @@ __device__ bool assign_item(const VolDev& v, const AssignCall& a, AssignLds& s, 
             const int na = i1 - i0, nb = m - na;
             const int j0 = (o0 - a_lo) - i0;
             for (int i = threadIdx.x; i < m; i += blockDim.x) {
-                const TouchEntry e = i < na ? A[i0 + i] : B[j0 + i - na];
-                s.key[i] = e.key;
-                s.slot[i] = e.slot;
+                const TouchEntry OSN_GLOBAL* e = i < na ? A + (i0 + i) : B + (j0 + i - na);
+                s.key[i] = e->key;
+                s.slot[i] = e->slot;
             }
             __syncthreads();
             const int per = (m + blockDim.x - 1) / blockDim.x;
