@@ -189,6 +189,89 @@ __device__ void load_cube(const VolDev& v, int32_t bi, float min_w, Cube<STORE>&
     __syncthreads();
 }
 
+// The count pass stages two blocks at once (the packed storages): their
+// neighbour indices together (lanes 0-7 and 8-15), then every corner load of
+// both in flight before the first wait, so two blocks cost one chain of two
+// dependent round trips. b1 < 0: one block.
+template <int STORE>
+__device__ void load_cube_pair(const VolDev& v, int32_t b0, int32_t b1, float min_w,
+                               Cube<STORE>* c, int32_t* nb_out, int32_t cached) {
+    static_assert(STORE != kStoreSums, "packed storages only");
+    const int tid = threadIdx.x;
+    if (tid < 16) {
+        const int k = tid & 7, w = tid >> 3;
+        const int32_t bi = w ? b1 : b0;
+        if (bi >= 0) {
+            int32_t nbi = bi;
+            const int32_t known = bi < cached ? global(nb_out)[(size_t)bi * 8 + k] : -1;
+            if (known >= 0) {
+                nbi = known;
+            } else if (k != 0) {
+                const int32_t bx = global(v.block_coord)[bi * 3 + 0] + (k & 1);
+                const int32_t by = global(v.block_coord)[bi * 3 + 1] + ((k >> 1) & 1);
+                const int32_t bz = global(v.block_coord)[bi * 3 + 2] + ((k >> 2) & 1);
+                nbi = find_index(v, pack_coord(bx, by, bz));
+            }
+            global(nb_out)[(size_t)bi * 8 + k] = nbi;
+            c[w].nb[k] = nbi;
+        }
+    }
+    __syncthreads();
+    using T = typename RawVox<STORE>::T;
+    const auto vox = global(static_cast<const T*>(v.vox));
+    constexpr int kRows = 81;
+    const int nw = b1 >= 0 ? 2 : 1;
+    T q[2][2][9];
+    bool ok_lo[2][2], ok_hi[2][2];
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        if (w >= nw) break;
+        const int32_t bi = w ? b1 : b0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = tid + h * kWaveThreads;
+            const int y = r % 9, z = r / 9;
+            const int o = ((y >> 3) << 1) | ((z >> 3) << 2);
+            const int32_t nlo = c[w].nb[o & 7], nhi = c[w].nb[(o | 1) & 7];
+            ok_lo[w][h] = nlo >= 0;
+            ok_hi[w][h] = nhi >= 0;
+            const int row = ((z & 7) * 8 + (y & 7)) * 8;
+            const auto plo = vox + (size_t)(ok_lo[w][h] ? nlo : bi) * kBlockVoxels + row;
+            const auto phi = vox + (size_t)(ok_hi[w][h] ? nhi : bi) * kBlockVoxels + row;
+            if (h == 0 || r < kRows) {
+#pragma unroll
+                for (int x = 0; x < 8; ++x) q[w][h][x] = plo[x];
+                q[w][h][8] = phi[0];
+            }
+        }
+    }
+    [[maybe_unused]] const uint32_t wfloor = f16_weight_floor(min_w);
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        if (w >= nw) break;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = tid + h * kWaveThreads;
+            if (h == 1 && r >= kRows) break;
+#pragma unroll
+            for (int x = 0; x < 9; ++x) {
+                const bool present = x < 8 ? ok_lo[w][h] : ok_hi[w][h];
+                if constexpr (STORE == kStoreF16) {
+                    c[w].val[r * 9 + x] = present && (q[w][h][x] >> 16) >= wfloor
+                                              ? (unsigned short)(q[w][h][x] & 0xffffu)
+                                              : (unsigned short)0x7e00u;
+                } else {
+                    const float wt = (float)q[w][h][x].y;
+                    c[w].val[r * 9 + x] = present && !(wt < min_w || wt <= 0.0f)
+                                              ? (float)q[w][h][x].x
+                                              : __builtin_nanf("");
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // The 6 tetrahedra (corner ids, 4 bits each, lowest first):
 // {0,5,1,7} {0,1,3,7} {0,3,2,7} {0,6,4,7} {0,4,5,7} {0,2,6,7}.
 // Kept as immediates: an array indexed by a lane-varying t would live in
@@ -441,20 +524,40 @@ __device__ inline int wg_exclusive_scan(int x, int* s_wave, int& total) {
 template <int STORE>
 __device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t first,
                              int32_t stride) {
-    __shared__ Cube<STORE> c;
+    [[maybe_unused]] __shared__ Cube<STORE> c;
     __shared__ __attribute__((aligned(16))) uint16_t s_table[256];
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
     const int32_t cached = global(v.ctr)->nb_blocks;
-    for (int32_t b = first; b < n; b += stride) {
-        load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb, cached);
-        uint32_t e[8];
-        uint32_t n3 = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e);
+    if constexpr (STORE != kStoreSums) {
+        // blocks b and b + stride together
+        __shared__ Cube<STORE> c2[2];
+        for (int32_t b = first; b < n; b += 2 * stride) {
+            const int32_t b1 = b + stride < n ? b + stride : -1;
+            load_cube_pair<STORE>(v, b, b1, a.min_weight, c2, a.block_nb, cached);
+            uint32_t e[8];
+            // two triangle counts (<= 6144 each) in one word, one reduction
+            uint32_t n3 = classify_row(c2[0], s_table, lane & 7, lane >> 3, a.iso, e);
+            if (b1 >= 0) n3 |= classify_row(c2[1], s_table, lane & 7, lane >> 3, a.iso, e) << 16;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) n3 += __shfl_xor(n3, o);
-        if (lane == 0) global(a.block_vcount)[b] = 3 * (int32_t)n3;
-        __syncthreads();  // the cube is read before the next block replaces it
+            for (int o = 32; o > 0; o >>= 1) n3 += __shfl_xor(n3, o);
+            if (lane == 0) {
+                global(a.block_vcount)[b] = 3 * (int32_t)(n3 & 0xffffu);
+                if (b1 >= 0) global(a.block_vcount)[b1] = 3 * (int32_t)(n3 >> 16);
+            }
+            __syncthreads();  // the cubes are read before the next pair replaces them
+        }
+    } else {
+        for (int32_t b = first; b < n; b += stride) {
+            load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb, cached);
+            uint32_t e[8];
+            uint32_t n3 = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) n3 += __shfl_xor(n3, o);
+            if (lane == 0) global(a.block_vcount)[b] = 3 * (int32_t)n3;
+            __syncthreads();  // the cube is read before the next block replaces it
+        }
     }
 }
 
@@ -677,15 +780,17 @@ hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipS
     // otherwise a grid-stride over the pool, four times the chip's worth
     const int32_t grid_blocks = a.n_blocks >= 0 ? (a.n_blocks > 0 ? a.n_blocks : 1)
                                                 : (v.pool_capacity < 32768 ? v.pool_capacity : 32768);
+    // count: two blocks per workgroup for the packed storages
+    const int32_t count_grid = v.storage == kStoreSums ? grid_blocks : (grid_blocks + 1) / 2;
     const int32_t n_for_scan = a.n_blocks >= 0 ? a.n_blocks : v.pool_capacity;
     const int32_t n_chunks = (n_for_scan + kScanChunk - 1) / kScanChunk;
     const int32_t scan_grid = n_chunks < 1 ? 1 : (n_chunks < kScanGrid ? n_chunks : kScanGrid);
     if (!emit) {
         switch (v.storage) {
             case kStoreSums: k_extract_count<kStoreSums><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
-            case kStoreF32: k_extract_count<kStoreF32><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
-            case kStoreF16: k_extract_count<kStoreF16><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
-            case kStoreF64: k_extract_count<kStoreF64><<<grid_blocks, kWaveThreads, 0, s>>>(v, a); break;
+            case kStoreF32: k_extract_count<kStoreF32><<<count_grid, kWaveThreads, 0, s>>>(v, a); break;
+            case kStoreF16: k_extract_count<kStoreF16><<<count_grid, kWaveThreads, 0, s>>>(v, a); break;
+            case kStoreF64: k_extract_count<kStoreF64><<<count_grid, kWaveThreads, 0, s>>>(v, a); break;
             default: return hipErrorInvalidValue;
         }
         k_extract_scan<<<scan_grid, kScanChunk, 0, s>>>(v, a);
@@ -707,13 +812,15 @@ hipError_t launch_extract_batch(const VolDev* vols, const ExtractArgs* args, int
     if (n_scenes <= 0 || grid_per_scene <= 0) return hipErrorInvalidValue;
     const uint32_t cap = max_pool_capacity > 0 ? max_pool_capacity : 1;
     const dim3 grid(cap < (uint32_t)grid_per_scene ? cap : (uint32_t)grid_per_scene, n_scenes);
+    // count: two blocks per workgroup
+    const dim3 count_grid((grid.x + 1) / 2, n_scenes);
     const uint32_t n_chunks = (cap + kScanChunk - 1) / kScanChunk;
     const dim3 scan_grid(n_chunks < kScanGridBatch ? n_chunks : kScanGridBatch, n_scenes);
     if (!emit) {
         switch (storage) {
-            case kStoreF32: k_extract_count_batch<kStoreF32><<<grid, kWaveThreads, 0, s>>>(vols, args); break;
-            case kStoreF16: k_extract_count_batch<kStoreF16><<<grid, kWaveThreads, 0, s>>>(vols, args); break;
-            case kStoreF64: k_extract_count_batch<kStoreF64><<<grid, kWaveThreads, 0, s>>>(vols, args); break;
+            case kStoreF32: k_extract_count_batch<kStoreF32><<<count_grid, kWaveThreads, 0, s>>>(vols, args); break;
+            case kStoreF16: k_extract_count_batch<kStoreF16><<<count_grid, kWaveThreads, 0, s>>>(vols, args); break;
+            case kStoreF64: k_extract_count_batch<kStoreF64><<<count_grid, kWaveThreads, 0, s>>>(vols, args); break;
             default: return hipErrorInvalidValue;
         }
         k_extract_scan_batch<<<scan_grid, kScanChunk, 0, s>>>(vols, args);
