@@ -875,6 +875,7 @@ struct OsnTsdfHipBatch {
     int32_t total_tiles = 0;
     uint32_t max_capacity = 0;
     uint32_t max_pool_capacity = 0;
+    int32_t max_w_max = 0;  // the largest projective weight cap of the volumes
     std::vector<int64_t> base_calls;
     int32_t alloc_grid = 0, update_grid = 0, update_threads = 256, lds_set = 1024;
     hipGraph_t graph = nullptr;
@@ -977,7 +978,8 @@ struct OsnTsdfHipBatch {
         hipError_t err = launch_alloc_batch(b, total_tiles, o, stream, ev[0]);
         if (err == hipSuccess) err = launch_assign_batch(b, max_capacity, stream, ev[1]);
         if (err == hipSuccess)
-            err = launch_proj_update_batch(b, storage, update_grid, update_threads, stream, ev[2]);
+            err = launch_proj_update_batch(b, storage, update_grid, update_threads, stream, ev[2],
+                                           max_w_max);
         if (timing)
             for (int i = 0; i < 3; ++i) ev_pending.push_back({i, {e[2 * i], e[2 * i + 1]}});
         return err;
@@ -1020,6 +1022,7 @@ OsnTsdfHipBatch* osn_tsdf_hip_batch_create(OsnTsdfHipVolume* const* volumes, int
         if (v->d.capacity > b->max_capacity) b->max_capacity = v->d.capacity;
         if ((uint32_t)v->d.pool_capacity > b->max_pool_capacity)
             b->max_pool_capacity = (uint32_t)v->d.pool_capacity;
+        if (v->d.w_max > b->max_w_max) b->max_w_max = v->d.w_max;
     }
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc(&b->d_vols, sizeof(VolDev) * n) == hipSuccess &&

@@ -133,9 +133,11 @@ hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStre
 hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, hipStream_t s);
 hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call, int32_t grid,
                               int32_t block_threads, hipStream_t s);
+// max_w_max: the largest weight cap of the batch's volumes (FP16 records
+// with caps below 64 take their averaging reciprocals from a table)
 hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t grid,
                                     int32_t block_threads, hipStream_t s,
-                                    const LaunchEvents& ev = {});
+                                    const LaunchEvents& ev = {}, int32_t max_w_max = 65535);
 hipError_t launch_step_advance(int32_t* step_ctr, hipStream_t s);
 hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipStream_t s);
 // Count + scan (emit false) or emit for every volume of a batch, one launch

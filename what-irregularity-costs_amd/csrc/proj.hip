@@ -119,6 +119,11 @@ __device__ inline void run_reciprocals(const float Z[4], float iz[4], float z[4]
     iz[0] = r1 * z[1];
 }
 
+// Weight caps below this many frames: the reciprocals 1 / (w + 1) of every
+// weight a record can hold come from a table, correctly rounded, instead of
+// the transcendental unit.
+constexpr int kRcpTable = 64;
+
 __device__ inline float avg_step(float t, float w, float x) {
     // (t * w + x) / (w + 1); the reciprocal's 1-ulp error is far inside the
     // storage rounding this feeds
@@ -202,9 +207,10 @@ __device__ inline Records<STORE> load_records(const Params& k, const BlockRef& b
 // out-of-image voxel reads pixel 0 and is then rejected), then ahead() runs,
 // which resolves the next block and requests its records, and only then does
 // the wave wait, for the gathers alone (loads complete in issue order).
-template <int STORE, typename Params, typename Ahead>
+template <int STORE, bool TAB, typename Params, typename Ahead>
 __device__ inline void update_block(const Params& k, const LaneRuns& L, const BlockRef& b,
-                                    const Records<STORE>& R, int lane, Ahead&& ahead) {
+                                    const Records<STORE>& R, int lane, const float* rtab,
+                                    Ahead&& ahead) {
     if (b.bi < 0) {
         ahead();
         return;
@@ -268,7 +274,13 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
                     const float x = fminf(sd[j] * k.inv_trunc, 1.0f);
                     const float tt = __half2float(__ushort_as_half((unsigned short)(word[j] & 0xffffu)));
                     const float w = (float)(word[j] >> 16);
-                    const float tn = avg_step(tt, w, x);
+                    float tn;
+                    if constexpr (TAB) {
+                        const uint32_t wi = word[j] >> 16;
+                        tn = __builtin_fmaf(tt, w, x) * rtab[wi < kRcpTable - 1 ? wi : kRcpTable - 1];
+                    } else {
+                        tn = avg_step(tt, w, x);
+                    }
                     const float wn = fminf(w + 1.0f, k.wmax);
                     const uint32_t nw = __builtin_amdgcn_perm(
                         (uint32_t)wn, (uint32_t)__half_as_ushort(__float2half_rn(tn)), 0x05040100u);
@@ -319,8 +331,13 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
 
 __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
-template <int STORE>
+template <int STORE, bool TAB>
 __global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, CallInfo call) {
+    [[maybe_unused]] __shared__ float s_rcp[TAB ? kRcpTable : 1];
+    if constexpr (TAB) {
+        if (threadIdx.x < kRcpTable) s_rcp[threadIdx.x] = 1.0f / (float)(threadIdx.x + 1);
+        __syncthreads();
+    }
     const int waves = blockDim.x >> 6;
     const int lane = threadIdx.x & 63;
     const UpdParams k = make_params(v, cam, call.parity);
@@ -340,7 +357,7 @@ __global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, Call
                     uint32_t& ent_fill) {
         const int next2 = next + stride;
         BlockRef nb{-1, 0, 0, 0};
-        update_block<STORE>(k, L, cur, use, lane, [&] {
+        update_block<STORE, TAB>(k, L, cur, use, lane, s_rcp, [&] {
             if (next < k.n) nb = resolve_block(k, ent);
             ent_fill = load_touch(k, next2 < k.n ? next2 : 0, lane);  // issued on every path
             fill = load_records<STORE>(k, nb, lane);
@@ -362,9 +379,11 @@ __global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, Call
 // the few scenes in its chunk.
 constexpr int kXcds = 8;
 
-template <int STORE>
+template <int STORE, bool TAB>
 __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
     __shared__ int s_prefix[kMaxBatchScenes + 1];
+    [[maybe_unused]] __shared__ float s_rcp[TAB ? kRcpTable : 1];
+    if (TAB && threadIdx.x < kRcpTable) s_rcp[threadIdx.x] = 1.0f / (float)(threadIdx.x + 1);
     const int waves = blockDim.x >> 6;
     const int lane = threadIdx.x & 63;
     const int ns = b.n_scenes;
@@ -405,7 +424,7 @@ __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
             lanes_sc = sc;
         }
         BlockRef nb{-1, 0, 0, 0};
-        update_block<STORE>(P[sc], L, cur, use, lane, [&] {
+        update_block<STORE, TAB>(P[sc], L, cur, use, lane, s_rcp, [&] {
             if (next < hi) nb = resolve_block(P[scn], ent);
             ent_fill = load_touch(P[scn2], next2 < hi ? next2 - s_prefix[scn2] : 0, lane);
             fill = load_records<STORE>(P[scn], nb, lane);
@@ -430,9 +449,14 @@ hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call
         block_threads != 64)
         return hipErrorInvalidValue;
     switch (v.storage) {
-        case kStoreF16: k_proj_update<kStoreF16><<<grid, block_threads, 0, s>>>(v, cam, call); break;
-        case kStoreF32: k_proj_update<kStoreF32><<<grid, block_threads, 0, s>>>(v, cam, call); break;
-        case kStoreF64: k_proj_update<kStoreF64><<<grid, block_threads, 0, s>>>(v, cam, call); break;
+        case kStoreF16:
+            if (v.w_max < kRcpTable)
+                k_proj_update<kStoreF16, true><<<grid, block_threads, 0, s>>>(v, cam, call);
+            else
+                k_proj_update<kStoreF16, false><<<grid, block_threads, 0, s>>>(v, cam, call);
+            break;
+        case kStoreF32: k_proj_update<kStoreF32, false><<<grid, block_threads, 0, s>>>(v, cam, call); break;
+        case kStoreF64: k_proj_update<kStoreF64, false><<<grid, block_threads, 0, s>>>(v, cam, call); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -440,7 +464,7 @@ hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call
 
 hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t grid,
                                     int32_t block_threads, hipStream_t s,
-                                    const LaunchEvents& ev) {
+                                    const LaunchEvents& ev, int32_t max_w_max) {
     if (b.n_scenes > kMaxBatchScenes) return hipErrorInvalidValue;
     if (grid <= 0) grid = 4096;
     grid = (grid + kXcds - 1) / kXcds * kXcds;
@@ -448,9 +472,14 @@ hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t 
         return hipErrorInvalidValue;
     if (block_threads < b.n_scenes) block_threads = 512;  // one thread per scene gathers
     switch (storage) {
-        case kStoreF16: launch_kernel(k_proj_update_batch<kStoreF16>, dim3(grid), dim3(block_threads), s, ev, b); break;
-        case kStoreF32: launch_kernel(k_proj_update_batch<kStoreF32>, dim3(grid), dim3(block_threads), s, ev, b); break;
-        case kStoreF64: launch_kernel(k_proj_update_batch<kStoreF64>, dim3(grid), dim3(block_threads), s, ev, b); break;
+        case kStoreF16:
+            if (max_w_max < kRcpTable)
+                launch_kernel(k_proj_update_batch<kStoreF16, true>, dim3(grid), dim3(block_threads), s, ev, b);
+            else
+                launch_kernel(k_proj_update_batch<kStoreF16, false>, dim3(grid), dim3(block_threads), s, ev, b);
+            break;
+        case kStoreF32: launch_kernel(k_proj_update_batch<kStoreF32, false>, dim3(grid), dim3(block_threads), s, ev, b); break;
+        case kStoreF64: launch_kernel(k_proj_update_batch<kStoreF64, false>, dim3(grid), dim3(block_threads), s, ev, b); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
