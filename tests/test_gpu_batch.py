@@ -199,6 +199,7 @@ def test_batched_extraction_equals_per_scene(H, S, use_graph):
     assert b.set_schedule(depths, frames) == 0
     caps = [1 << 25, 1 << 25, 64]  # the last scene's mesh does not fit
     bufs = [torch.empty(c * 6, dtype=torch.float32, device="cuda") for c in caps]
+    bufs[2].fill_(7.0)  # an overflowing extraction writes nothing
     counts = torch.full((3,), -7, dtype=torch.int32, device="cuda")
     assert b.set_extract(bufs, caps, counts, 0.5, 0.0, every=3) == 0
     for k in range(n_steps):
@@ -211,6 +212,7 @@ def test_batched_extraction_equals_per_scene(H, S, use_graph):
             continue
         got = counts.cpu().numpy()
         assert got[2] == -1
+        assert bool(torch.all(bufs[2] == 7.0))
         for s in range(2):
             ref = torch.empty(caps[s] * 6, dtype=torch.float32, device="cuda")
             rc = torch.zeros(1, dtype=torch.int32, device="cuda")
