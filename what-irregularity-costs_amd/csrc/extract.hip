@@ -1,18 +1,21 @@
 // Marching-tetrahedra extraction as count -> scan -> emit, with no atomics on
-// vertex placement: every vertex lands at (block offset + voxel offset within
-// the block + tetrahedron order), so the same volume state always yields the
-// same bytes.
+// vertex placement: a block's vertices start at the scan of the per-block
+// counts and its triangles follow in (voxel, tetrahedron, quad half) order, so
+// the same volume state always yields the same bytes.
 //
 // Per cube semantics follow cpp/src/marching_tets.cpp:84-243: a cube per voxel
 // spanning (v..v+1)^3, corners sampled at voxel centres, any invalid corner
 // (absent block, w < min_weight or w <= 0) skips the cube, 6 tetrahedra sharing
 // the 0-7 diagonal, position-ordered interpolation, one face normal per
 // triangle. The reference launches one thread per voxel and performs 8 hash
-// lookups per voxel (cpp/src/marching_tets_cuda.cu:73-167); here a workgroup
-// stages the block's 9x9x9 corner samples in LDS and classifies from there.
-// The count pass looks the 7 neighbour blocks up once per block and keeps
-// their indices for the emit pass, which assembles each round's vertices in
-// LDS and writes them out as one contiguous, coalesced run.
+// lookups per voxel (cpp/src/marching_tets_cuda.cu:73-167). Here one wave
+// handles a block (the count pass two): it stages the block's 9x9x9 corner
+// samples in LDS, each lane classifies a row of 8 cubes through a mask table,
+// and the emit pass lists the block's triangles and emits one per lane in
+// groups of 64, assembled in LDS and written out as one contiguous run. The
+// count pass looks the 7 neighbour blocks up and keeps their indices, for the
+// emit pass and for later count passes (pool indices never move until a
+// reset). A batch runs each pass as one launch for all its volumes.
 //
 // Compiled with -ffp-contract=off so vertex arithmetic matches the reference
 // CPU extractor's rounding.
@@ -608,7 +611,7 @@ __device__ void scan_chunks(const VolDev& v, const ExtractArgs& a) {
                 a.block_voff[first + k] = run;
                 run += a.block_vcount[first + k];
             }
-        if (threadIdx.x < n_chunks) a.chunk_sums[threadIdx.x] = 0;
+        if ((int)threadIdx.x < n_chunks) a.chunk_sums[threadIdx.x] = 0;
         if (threadIdx.x == 0) {
             const int32_t out = total > a.cap ? -1 : total;
             *a.count_out = out;
