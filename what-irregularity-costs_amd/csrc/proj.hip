@@ -52,6 +52,7 @@ namespace {
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 template <typename T>
 __device__ inline T OSN_GLOBAL* glob(T* p) {
@@ -269,6 +270,7 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
         if constexpr (STORE == kStoreF16) {
             if (o[0] | o[1] | o[2] | o[3]) {
                 uint32_t word[4] = {R.h[r].x, R.h[r].y, R.h[r].z, R.h[r].w};
+                const unsigned short wcap = (unsigned short)k.wcap;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const float x = fminf(sd[j] * k.inv_trunc, 1.0f);
@@ -281,10 +283,13 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
                     } else {
                         tn = avg_step(tt, w, x);
                     }
-                    const float wn = fminf(w + 1.0f, k.wmax);
-                    const uint32_t nw = __builtin_amdgcn_perm(
-                        (uint32_t)wn, (uint32_t)__half_as_ushort(__float2half_rn(tn)), 0x05040100u);
-                    word[j] = o[j] ? nw : word[j];
+                    // the weight min(w + 1, w_max) in the high half (saturating
+                    // 16-bit add, then the cap), the new mean in the low half
+                    u16x2 pw = __builtin_bit_cast(u16x2, word[j]);
+                    pw = __builtin_elementwise_add_sat(pw, u16x2{0, 1});
+                    pw = __builtin_elementwise_min(pw, u16x2{0xffff, wcap});
+                    pw.x = __half_as_ushort(__float2half_rn(tn));
+                    word[j] = o[j] ? __builtin_bit_cast(uint32_t, pw) : word[j];
                 }
                 u32x4 OSN_GLOBAL* rec = reinterpret_cast<u32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 4);
                 rec[r * 64] = u32x4{word[0], word[1], word[2], word[3]};
@@ -372,29 +377,37 @@ __global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, Call
 }
 
 // All scenes of a batched step in one launch: the work list is the
-// concatenation of every active scene's touched blocks. Each workgroup gathers
-// the scenes' parameters and the prefix of their touched counts into LDS. The
-// list is cut into one contiguous chunk per XCD (workgroups are dealt to the
-// 8 XCDs round-robin by id), so each XCD's L2 serves the depth images of only
-// the few scenes in its chunk.
+// concatenation of every active scene's touched blocks. Each wave holds the
+// prefix of the scenes' touched counts in a register (lane s: the first item
+// of scene s), so the scene of an item is one ballot. The list is cut into one
+// contiguous chunk per XCD (workgroups are dealt to the 8 XCDs round-robin by
+// id), so each XCD's L2 serves the depth images of only the few scenes in its
+// chunk.
 constexpr int kXcds = 8;
+static_assert(kMaxBatchScenes <= 64, "one lane per scene");
 
 template <int STORE, bool TAB>
 __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
-    __shared__ int s_prefix[kMaxBatchScenes + 1];
     [[maybe_unused]] __shared__ float s_rcp[TAB ? kRcpTable : 1];
-    if (TAB && threadIdx.x < kRcpTable) s_rcp[threadIdx.x] = 1.0f / (float)(threadIdx.x + 1);
+    if constexpr (TAB) {
+        if (threadIdx.x < kRcpTable) s_rcp[threadIdx.x] = 1.0f / (float)(threadIdx.x + 1);
+        __syncthreads();
+    }
     const int waves = blockDim.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int ns = b.n_scenes;
     const UpdParams OSN_CONST* P = (const UpdParams OSN_CONST*)b.upd;
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int sc = 0; sc < ns; ++sc) { s_prefix[sc] = acc; acc += P[sc].n; }
-        s_prefix[ns] = acc;
+    const int cnt = lane < b.n_scenes ? P[lane].n : 0;
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
     }
-    __syncthreads();
-    const int n = s_prefix[ns];
+    // lanes past the last scene hold n, above every item
+    const int first = incl - cnt;
+    const int n = __builtin_amdgcn_readlane(incl, 63);
+    auto scene_of = [&](int x) { return (int)__popcll(__ballot(first <= x)) - 1; };
+    auto first_of = [&](int sc) { return __builtin_amdgcn_readlane(first, sc); };
     const int xcd = blockIdx.x % kXcds;
     const int per_xcd = gridDim.x / kXcds;  // the launcher keeps the grid a multiple of 8
     const int lo = (int)((long long)n * xcd / kXcds);
@@ -402,23 +415,20 @@ __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
     const int stride = per_xcd * waves;
     const int item = lo + (int)(blockIdx.x / kXcds) * waves + wave_id();
     if (item >= hi) return;
-    // Pipeline as in k_proj_update. Items ascend, so the scene of the next
-    // items only moves forward.
-    int sc = 0;
-    while (s_prefix[sc + 1] <= item) ++sc;
-    BlockRef cur = resolve_block(P[sc], load_touch(P[sc], item - s_prefix[sc], lane));
+    // Pipeline as in k_proj_update.
+    int sc = scene_of(item);
+    BlockRef cur = resolve_block(P[sc], load_touch(P[sc], item - first_of(sc), lane));
     Records<STORE> ra = load_records<STORE>(P[sc], cur, lane), rb{};
-    int next = item + stride, scn = sc;
-    while (next < hi && s_prefix[scn + 1] <= next) ++scn;
+    int next = item + stride;
+    int scn = next < hi ? scene_of(next) : sc;
     // entry 0 of this scene stands in for an absent next entry (never used)
-    uint32_t ea = load_touch(P[scn], next < hi ? next - s_prefix[scn] : 0, lane), eb = 0u;
+    uint32_t ea = load_touch(P[scn], next < hi ? next - first_of(scn) : 0, lane), eb = 0u;
     int lanes_sc = -1;
     LaneRuns L{};
     auto pass = [&](const Records<STORE>& use, Records<STORE>& fill, uint32_t ent,
                     uint32_t& ent_fill) {
         const int next2 = next + stride;
-        int scn2 = scn;
-        while (next2 < hi && s_prefix[scn2 + 1] <= next2) ++scn2;
+        const int scn2 = next2 < hi ? scene_of(next2) : scn;
         if (sc != lanes_sc) {  // the lane offsets depend on the scene's pose
             L = lane_runs(P[sc], lane);
             lanes_sc = sc;
@@ -426,7 +436,7 @@ __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
         BlockRef nb{-1, 0, 0, 0};
         update_block<STORE, TAB>(P[sc], L, cur, use, lane, s_rcp, [&] {
             if (next < hi) nb = resolve_block(P[scn], ent);
-            ent_fill = load_touch(P[scn2], next2 < hi ? next2 - s_prefix[scn2] : 0, lane);
+            ent_fill = load_touch(P[scn2], next2 < hi ? next2 - first_of(scn2) : 0, lane);
             fill = load_records<STORE>(P[scn], nb, lane);
         });
         cur = nb;
@@ -470,7 +480,6 @@ hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t 
     grid = (grid + kXcds - 1) / kXcds * kXcds;
     if (block_threads != 256 && block_threads != 128 && block_threads != 512)
         return hipErrorInvalidValue;
-    if (block_threads < b.n_scenes) block_threads = 512;  // one thread per scene gathers
     switch (storage) {
         case kStoreF16:
             if (max_w_max < kRcpTable)

@@ -55,6 +55,7 @@ struct UpdParams {
     float fx, fy, cxh, cyh;
     int32_t W, H;
     float voxel, trunc, inv_trunc, wmax;
+    int32_t wcap;       // wmax as an integer, 1..65535
     int32_t n;          // touched blocks
     int32_t tile_k;     // tiled_stride_k(W)
     double Rcw64[9];    // FP64 oracle kernel only
@@ -90,6 +91,7 @@ __device__ inline UpdParams make_params(const VolDev& v, const ProjCam& k, int p
     p.trunc = v.trunc;
     p.inv_trunc = 1.0f / v.trunc;
     p.wmax = (float)v.w_max;
+    p.wcap = v.w_max;
     p.n = v.ctr->lists[parity].touched_count;
     p.tile_k = tiled_stride_k(k.W);
     return p;
