@@ -148,10 +148,11 @@ __device__ inline u32x4 touch_entry(uint32_t part) {
 }
 
 // A touched block, uniform over the wave: its pool index (negative: dropped,
-// nothing to update) and block coordinates.
+// nothing to update), block coordinates and block-origin forms (A0, B0, Z0).
 struct BlockRef {
     int32_t bi;
     int32_t bx, by, bz;
+    float f0[3];
 };
 
 template <typename Params>
@@ -161,6 +162,13 @@ __device__ inline BlockRef resolve_block(const Params& k, uint32_t part) {
     b.bi = (int32_t)e.w;
     if (b.bi == kIdxPending) b.bi = glob(k.table)[e.z].block_idx;  // inserted by this call
     unpack_key((int64_t)(((uint64_t)e.y << 32) | e.x), b.bx, b.by, b.bz);
+    // the forms are computed here, while the previous block's gathers are in
+    // flight, and kept in scalar registers
+    float f0[3];
+    block_forms(k, b.bx, b.by, b.bz, f0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        b.f0[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(f0[i])));
     return b;
 }
 
@@ -219,8 +227,7 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
     const int32_t bx = b.bx, by = b.by, bz = b.bz;
     const size_t v0 = (size_t)b.bi * kBlockVoxels + lane * 4;  // first voxel of run 0
 
-    float f0[3];
-    block_forms(k, bx, by, bz, f0);
+    const float* f0 = b.f0;
     float Zv[8];
     uint32_t pix[8];
     bool inb[8];
@@ -361,7 +368,7 @@ __global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, Call
     auto pass = [&](const Records<STORE>& use, Records<STORE>& fill, uint32_t ent,
                     uint32_t& ent_fill) {
         const int next2 = next + stride;
-        BlockRef nb{-1, 0, 0, 0};
+        BlockRef nb{-1, 0, 0, 0, {0.0f, 0.0f, 0.0f}};
         update_block<STORE, TAB>(k, L, cur, use, lane, s_rcp, [&] {
             if (next < k.n) nb = resolve_block(k, ent);
             ent_fill = load_touch(k, next2 < k.n ? next2 : 0, lane);  // issued on every path
@@ -433,7 +440,7 @@ __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
             L = lane_runs(P[sc], lane);
             lanes_sc = sc;
         }
-        BlockRef nb{-1, 0, 0, 0};
+        BlockRef nb{-1, 0, 0, 0, {0.0f, 0.0f, 0.0f}};
         update_block<STORE, TAB>(P[sc], L, cur, use, lane, s_rcp, [&] {
             if (next < hi) nb = resolve_block(P[scn], ent);
             ent_fill = load_touch(P[scn2], next2 < hi ? next2 - first_of(scn2) : 0, lane);
