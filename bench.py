@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE x 1.0). Those launches are steps 0..29 of the same 16 scenes, so their
 # algorithmic bytes are given beside them; counters cannot run inside this
 # timed benchmark.
-UPDATE_TRAFFIC = {"hbm_bytes_per_launch": 395.5e6, "read": 238.7e6, "write": 156.8e6,
+UPDATE_TRAFFIC = {"hbm_bytes_per_launch": 395.4e6, "read": 238.6e6, "write": 156.8e6,
                   "algorithmic_bytes_per_launch": 432.9e6,
                   "launches": "k_proj_update_batch<fp16>, 16 scenes, steps 0..29, "
                               "100.6k touched blocks per launch",
