@@ -172,14 +172,20 @@ def test_fp64_kernel_equals_cpu_fp64_oracle(H, S, W, Hh, separate):
     assert n_obs > 10000
 
 
-def test_fp32_and_fp16_against_fp64(H, S):
+# Weight cap 32: voxels seen in more than 32 of the 40 frames saturate, and the
+# FP16 kernel takes its averaging reciprocals from the table (caps below 64);
+# cap 64 runs the FP16 kernel without the table.
+@pytest.mark.parametrize("wcap", [32.0, 64.0])
+def test_fp32_and_fp16_against_fp64(H, S, wcap):
     cfg = dict(S.PRECISE, table=1 << 20)
     _, fr = frames_for(H, S, cfg, 21, 40)
-    ref = _run(H, H.STORE_F64, cfg, fr, 32.0)
+    ref = _run(H, H.STORE_F64, cfg, fr, wcap)
     k64 = ref.keys()
     t64, w64 = ref.read_voxels_f64()
+    if wcap == 32.0:
+        assert (w64 == 32.0).sum() > 1000  # the cap is reached
     for storage, tol in ((H.STORE_F32, 1e-4), (H.STORE_F16, 0.01)):
-        v = _run(H, storage, cfg, fr, 32.0)
+        v = _run(H, storage, cfg, fr, wcap)
         k, t, w = _by_key(v)
         assert np.array_equal(k, k64)  # same sorted-key pool order
         assert np.array_equal(w.astype(np.float64), w64)
