@@ -1217,8 +1217,6 @@ __global__ void k_backproject(PointSrc src, float* out) {
     o[2] = src.R[6] * xc + src.R[7] * yc + src.R[8] * z + src.cam[2];
 }
 
-__global__ void k_step_advance(int32_t* step) { *step = *step + 1; }
-
 }  // namespace
 
 template <int MODE, bool TOUCH>
@@ -1324,11 +1322,6 @@ hipError_t launch_tile_depth(const float* depth, int32_t W, int32_t H, float* ti
 hipError_t launch_backproject(const PointSrc& src, float* out, hipStream_t s) {
     dim3 grid((src.W + 15) / 16, (src.H + 15) / 16);
     k_backproject<<<grid, 256, 0, s>>>(src, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_step_advance(int32_t* step_ctr, hipStream_t s) {
-    k_step_advance<<<1, 1, 0, s>>>(step_ctr);
     return hipGetLastError();
 }
 

@@ -934,7 +934,6 @@ struct OsnTsdfHipBatch {
         }
         hipError_t err = enqueue_step(dev(0, true));
         if (err == hipSuccess && with_extract) err = enqueue_extract();
-        if (err == hipSuccess) err = launch_step_advance(d_step, stream);
         hipGraph_t g = nullptr;
         const hipError_t end = hipStreamEndCapture(stream, &g);
         timing = t;
@@ -955,6 +954,7 @@ struct OsnTsdfHipBatch {
         b.steps = d_steps;
         b.tile_prefix = d_tile_prefix;
         b.step_ctr = from_device ? d_step : nullptr;
+        b.step_advance = from_device ? d_step : nullptr;  // the update kernel advances it
         b.call_base = d_call_base;
         b.upd = d_upd;
         b.step = step;

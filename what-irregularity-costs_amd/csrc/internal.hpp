@@ -64,12 +64,15 @@ constexpr int kMaxBatchScenes = 64;  // volumes per batch
 
 // A batched step over many scenes, read from device memory so that a captured
 // graph can replay it: scene s of step k is steps[k * n_scenes + s], with k
-// read from *step_ctr when step_ctr is set, else `step`.
+// read from *step_ctr when step_ctr is set, else `step`. When step_advance is
+// set, the step's update kernel (which runs after every reader of the
+// counter in the step) adds one to it for the next replay.
 struct BatchDev {
     const VolDev* vols;
     const SceneStep* steps;
     const int32_t* tile_prefix;  // n_scenes + 1, exclusive prefix of tiles
     const int32_t* step_ctr;
+    int32_t* step_advance;
     const int32_t* call_base;    // per scene: calls the volume had made before step 0
     UpdParams* upd;              // per scene: the update's parameters, written each step
     int32_t step;
@@ -138,7 +141,6 @@ hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call
 hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t grid,
                                     int32_t block_threads, hipStream_t s,
                                     const LaunchEvents& ev = {}, int32_t max_w_max = 65535);
-hipError_t launch_step_advance(int32_t* step_ctr, hipStream_t s);
 hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipStream_t s);
 // Count + scan (emit false) or emit for every volume of a batch, one launch
 // each: blockIdx.y is the volume, grid_per_scene one-wave workgroups walk its

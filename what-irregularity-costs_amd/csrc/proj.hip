@@ -400,6 +400,9 @@ __global__ __launch_bounds__(512) void k_proj_update_batch(BatchDev b) {
         if (threadIdx.x < kRcpTable) s_rcp[threadIdx.x] = 1.0f / (float)(threadIdx.x + 1);
         __syncthreads();
     }
+    // the step's allocate and assign kernels, the only readers of the step
+    // counter, have finished: advance it for the next replay
+    if (b.step_advance && blockIdx.x == 0 && threadIdx.x == 0) *glob(b.step_advance) += 1;
     const int waves = blockDim.x >> 6;
     const int lane = threadIdx.x & 63;
     const UpdParams OSN_CONST* P = (const UpdParams OSN_CONST*)b.upd;
