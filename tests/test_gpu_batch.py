@@ -93,6 +93,44 @@ def test_batched_equals_per_scene(H, S, storage):
         assert same(state(v1), state(v2))
 
 
+# Twelve small scenes: scene 1 mod 4 idles on odd steps and scene 5 never has
+# a frame, so scenes with no touched blocks sit in the middle of the batch's
+# work list as well as at its end.
+def test_batched_many_scenes_with_idle_ones(H, S):
+    n_steps, W, Hh, n_sc = 4, 160, 120, 12
+    cfg = S.PRECISE
+    fx, fy, cx, cy = S.intrinsics(W, Hh)
+    depths = [[None] * n_sc for _ in range(n_steps)]
+    frames = [[None] * n_sc for _ in range(n_steps)]
+    for s in range(n_sc):
+        scene, poses = S.build(dict(cfg, frames=n_steps), 70 + s)
+        sc = scene.to_c()
+        for k, (R, t) in enumerate(poses):
+            if s == 5 or (s % 4 == 1 and k % 2 == 1):
+                continue
+            fr = H.make_frame(W, Hh, fx, fy, cx, cy, R, t)
+            d = torch.empty(W * Hh, dtype=torch.float32, device="cuda")
+            assert S.render_depth(sc, fr, cfg["noise_k"], cfg["dropout"], 70 + s, k, d) == 0
+            depths[k][s] = d
+            frames[k][s] = fr
+    torch.cuda.synchronize()
+    solo = volumes(H, 2, tables=[1 << 16] * n_sc)
+    for k in range(n_steps):
+        for s, v in enumerate(solo):
+            if depths[k][s] is not None:
+                assert v.integrate_depth(depths[k][s], frames[k][s]) == 0
+    batched = volumes(H, 2, tables=[1 << 16] * n_sc)
+    b = H.Batch(batched)
+    assert b.set_schedule(depths, frames) == 0
+    assert b.run(n_steps, use_graph=True) == 0
+    b.synchronize()
+    for s, (v1, v2) in enumerate(zip(solo, batched)):
+        assert v1.block_count == v2.block_count
+        assert (v1.block_count > 0) == (s != 5)
+        assert same(state(v1), state(v2)), f"scene {s}"
+    b.close()
+
+
 def test_graph_replay_matches_eager_every_step(H, S):
     n_steps = 6
     depths, frames = make_inputs(H, S, n_steps, seed0=70)
