@@ -93,3 +93,74 @@ def test_frame_descriptor_casts_once():
     assert list(f.t_wc64) == list(t)
     assert np.float32(f.t_wc[2]) == np.float32(1.0 / 3.0)
     assert f.width == 640 and f.height == 480
+
+
+def test_autotuner_scores_update_shapes_over_two_rounds():
+    """The tuner's search logic on a scripted stand-in for the batch: the
+    update shape is chosen by its mean over the interleaved rounds (a shape
+    that is fast once by chance loses), and 2048-workgroup grids are not tried."""
+    import autotune
+
+    script = {}  # (threads, grid) -> update ms per round
+
+    n_alloc_trials = len(autotune.ALLOC_GRIDS) * len(autotune.LDS_SETS) * len(autotune.HASHES)
+
+    class FakeBatch:
+        calls = {}
+        n = 0
+
+        def __init__(self, vols):
+            self.k = None
+            self.reads = 0
+
+        def tune(self, alloc_grid, update_grid, update_threads, lds_set):
+            self.k = (alloc_grid, lds_set, update_threads, update_grid)
+            return 0
+
+        def set_schedule(self, depths, frames):
+            return 0
+
+        def run(self, n):
+            return 0
+
+        def synchronize(self):
+            pass
+
+        def set_timing(self, on):
+            pass
+
+        def read_timing(self):
+            alloc_grid, lds_set, t, g = self.k
+            self.reads += 1
+            if self.reads == 1:  # the read that clears the warmup's timings
+                return np.zeros(3), np.zeros(3, np.int64)
+            FakeBatch.n += 1
+            upd = 0.09
+            if FakeBatch.n > n_alloc_trials:  # the update phase: count rounds per shape
+                i = FakeBatch.calls.get((t, g), 0)
+                FakeBatch.calls[(t, g)] = i + 1
+                upd = script.get((t, g), [0.09, 0.09])[i]
+            alloc = 0.09 - 0.001 * (alloc_grid == 1024) - 0.0005 * (lds_set == 256)
+            return np.array([alloc, 0.01, upd]), np.array([1, 1, 1])
+
+        def close(self):
+            pass
+
+    class FakeH:
+        Batch = FakeBatch
+
+    class Vol:
+        def close(self):
+            pass
+
+    # (256, 4096) is fastest once and slow once; (128, 8192) is steady
+    script[(256, 4096)] = [0.070, 0.095]
+    script[(128, 8192)] = [0.080, 0.080]
+    best, log = autotune.tune(FakeH, lambda hk: [Vol()], [[None]], [[None]], warmup=1, steps=1)
+    assert (best["update_threads"], best["update_grid"]) == (128, 8192)
+    assert best["alloc_grid"] == 1024 and best["lds_set"] == 256
+    tried = {(t["knobs"]["update_threads"], t["knobs"]["update_grid"]) for t in log["trials"]}
+    assert all(g != 2048 for _, g in tried)
+    n_update_trials = len(autotune.UPDATE_THREADS) * len(autotune.UPDATE_GRIDS)
+    assert len(log["trials"]) == n_alloc_trials + autotune.UPDATE_ROUNDS * n_update_trials
+    assert FakeBatch.calls[(256, 4096)] == autotune.UPDATE_ROUNDS == 2
