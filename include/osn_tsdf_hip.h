@@ -136,7 +136,9 @@ typedef struct {
     float voxel_size_m;
     int32_t pool_capacity_blocks;
     float trunc_m;          /* <= 0: 4 * voxel */
-    float weight_cap;       /* <= 0: no cap (projective storage saturates at 65535) */
+    float weight_cap;       /* a voxel stops changing once its weight reaches the cap
+                               (types.hpp:99-101); <= 0: no cap (projective storage
+                               freezes at the u16 limit 65535) */
     int32_t storage;        /* OSN_TSDF_HIP_STORE_* */
     uint32_t table_slots;   /* 0: next_pow2(2 * pool) */
     int32_t hash_kind;      /* OSN_TSDF_HIP_HASH_*; REFERENCE needs a power of two */

@@ -48,71 +48,80 @@ def lib():
     if _lib is None:
         if not os.path.exists(ORACLE_SO):
             build()
-        L = C.CDLL(ORACLE_SO)
-        L.orc_create.restype = C.c_void_p
-        L.orc_create.argtypes = [C.c_float, C.c_int32, C.c_float, C.c_float]
-        L.orc_destroy.argtypes = [C.c_void_p]
-        for name in ("orc_allocate_blocks",):
-            getattr(L, name).argtypes = [C.c_void_p, f32p, C.c_int32, C.c_float, C.c_float,
-                                         C.c_float, C.c_float]
-            getattr(L, name).restype = C.c_int32
-        for name in ("orc_update_voxels", "orc_integrate"):
-            getattr(L, name).argtypes = [C.c_void_p, f32p, u8p, f32p, C.c_int32, C.c_float,
-                                         C.c_float, C.c_float, C.c_float]
-            getattr(L, name).restype = C.c_int32
-        L.orc_count_visits.argtypes = [C.c_void_p, f32p, C.c_int32, C.c_float, C.c_float,
-                                       C.c_float, C.c_float]
-        L.orc_count_visits.restype = C.c_int64
-        L.orc_touched_keys.argtypes = [C.c_void_p, f32p, C.c_int32, C.c_float, C.c_float,
-                                       C.c_float, C.c_float, i64p, C.c_int32]
-        L.orc_touched_keys.restype = C.c_int32
-        L.orc_extract_mesh.argtypes = [C.c_void_p, f32p, u8p, C.c_int32, C.c_float, C.c_float]
-        L.orc_extract_mesh.restype = C.c_int32
-        L.orc_block_count.argtypes = [C.c_void_p]
-        L.orc_block_count.restype = C.c_int32
-        L.orc_drop_count.argtypes = [C.c_void_p]
-        L.orc_drop_count.restype = C.c_uint64
-        L.orc_table_size.argtypes = [C.c_void_p]
-        L.orc_table_size.restype = C.c_uint32
-        L.orc_trunc.argtypes = [C.c_void_p]
-        L.orc_trunc.restype = C.c_float
-        L.orc_block_keys.argtypes = [C.c_void_p, i64p, C.c_int32]
-        L.orc_block_keys.restype = C.c_int32
-        L.orc_find_key.argtypes = [C.c_void_p, C.c_int64]
-        L.orc_find_key.restype = C.c_int32
-        L.orc_block_voxels.argtypes = [C.c_void_p, C.c_int32, f32p, f32p, f32p, f32p, f32p]
-        L.orc_block_voxels.restype = C.c_int32
-        L.orc_load_block.argtypes = [C.c_void_p, C.c_int64, f32p, f32p, C.c_int32]
-        L.orc_load_block.restype = C.c_int32
-        L.orc_pack_coord.argtypes = [C.c_int32] * 3
-        L.orc_pack_coord.restype = C.c_int64
-        L.orc_unpack_key.argtypes = [C.c_int64, i32p]
-        L.orc_hash_block.argtypes = [C.c_int32] * 3 + [C.c_uint32]
-        L.orc_hash_block.restype = C.c_uint32
-        L.orc_floor_div.argtypes = [C.c_int32, C.c_int32]
-        L.orc_floor_div.restype = C.c_int32
-        L.orc_scene_sphere.argtypes = [C.c_float, C.c_int32, C.c_int32, f32p]
-        L.orc_scene_plane.argtypes = [C.c_float, C.c_float, C.c_int32, f32p]
-        L.orc_backproject_f32.argtypes = [f32p, C.c_int32, C.c_int32, C.c_float, C.c_float,
-                                          C.c_float, C.c_float, f32p, f32p, C.c_float, f32p]
-        L.orc_backproject_f32.restype = C.c_int32
-        L.orc_proj_observe_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_float,
-                                           C.c_float, C.c_int32, C.c_int32, f32p, C.c_float,
-                                           C.c_float] + [C.c_int32] * 6 + [f32p]
-        L.orc_proj_observe_f32.restype = C.c_int32
-        L.orc_proj_create.restype = C.c_void_p
-        L.orc_proj_create.argtypes = [C.c_float, C.c_float, C.c_int32, C.c_int32]
-        L.orc_proj_destroy.argtypes = [C.c_void_p]
-        L.orc_proj_integrate.argtypes = [C.c_void_p, i64p, C.c_int32, f32p, C.c_int32,
-                                         C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
-                                         f64p, f64p, f32p, f32p]
-        L.orc_proj_integrate.restype = C.c_int64
-        L.orc_proj_block_count.argtypes = [C.c_void_p]
-        L.orc_proj_block_count.restype = C.c_int32
-        L.orc_proj_block.argtypes = [C.c_void_p, C.c_int64, f64p, i32p]
-        L.orc_proj_block.restype = C.c_int32
-        _lib = L
+        _lib = _declare(C.CDLL(ORACLE_SO))
     return _lib
+
+
+def load_variant(path: str):
+    """A separately built copy of the oracle (for instance one compiled with a
+    planted mistake, -DORC_PLANT=n), with the same signatures declared."""
+    return _declare(C.CDLL(path))
+
+
+def _declare(L):
+    L.orc_create.restype = C.c_void_p
+    L.orc_create.argtypes = [C.c_float, C.c_int32, C.c_float, C.c_float]
+    L.orc_destroy.argtypes = [C.c_void_p]
+    for name in ("orc_allocate_blocks",):
+        getattr(L, name).argtypes = [C.c_void_p, f32p, C.c_int32, C.c_float, C.c_float,
+                                     C.c_float, C.c_float]
+        getattr(L, name).restype = C.c_int32
+    for name in ("orc_update_voxels", "orc_integrate"):
+        getattr(L, name).argtypes = [C.c_void_p, f32p, u8p, f32p, C.c_int32, C.c_float,
+                                     C.c_float, C.c_float, C.c_float]
+        getattr(L, name).restype = C.c_int32
+    L.orc_count_visits.argtypes = [C.c_void_p, f32p, C.c_int32, C.c_float, C.c_float,
+                                   C.c_float, C.c_float]
+    L.orc_count_visits.restype = C.c_int64
+    L.orc_touched_keys.argtypes = [C.c_void_p, f32p, C.c_int32, C.c_float, C.c_float,
+                                   C.c_float, C.c_float, i64p, C.c_int32]
+    L.orc_touched_keys.restype = C.c_int32
+    L.orc_extract_mesh.argtypes = [C.c_void_p, f32p, u8p, C.c_int32, C.c_float, C.c_float]
+    L.orc_extract_mesh.restype = C.c_int32
+    L.orc_block_count.argtypes = [C.c_void_p]
+    L.orc_block_count.restype = C.c_int32
+    L.orc_drop_count.argtypes = [C.c_void_p]
+    L.orc_drop_count.restype = C.c_uint64
+    L.orc_table_size.argtypes = [C.c_void_p]
+    L.orc_table_size.restype = C.c_uint32
+    L.orc_trunc.argtypes = [C.c_void_p]
+    L.orc_trunc.restype = C.c_float
+    L.orc_block_keys.argtypes = [C.c_void_p, i64p, C.c_int32]
+    L.orc_block_keys.restype = C.c_int32
+    L.orc_find_key.argtypes = [C.c_void_p, C.c_int64]
+    L.orc_find_key.restype = C.c_int32
+    L.orc_block_voxels.argtypes = [C.c_void_p, C.c_int32, f32p, f32p, f32p, f32p, f32p]
+    L.orc_block_voxels.restype = C.c_int32
+    L.orc_load_block.argtypes = [C.c_void_p, C.c_int64, f32p, f32p, C.c_int32]
+    L.orc_load_block.restype = C.c_int32
+    L.orc_pack_coord.argtypes = [C.c_int32] * 3
+    L.orc_pack_coord.restype = C.c_int64
+    L.orc_unpack_key.argtypes = [C.c_int64, i32p]
+    L.orc_hash_block.argtypes = [C.c_int32] * 3 + [C.c_uint32]
+    L.orc_hash_block.restype = C.c_uint32
+    L.orc_floor_div.argtypes = [C.c_int32, C.c_int32]
+    L.orc_floor_div.restype = C.c_int32
+    L.orc_scene_sphere.argtypes = [C.c_float, C.c_int32, C.c_int32, f32p]
+    L.orc_scene_plane.argtypes = [C.c_float, C.c_float, C.c_int32, f32p]
+    L.orc_backproject_f32.argtypes = [f32p, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                      C.c_float, C.c_float, f32p, f32p, C.c_float, f32p]
+    L.orc_backproject_f32.restype = C.c_int32
+    L.orc_proj_observe_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_float,
+                                       C.c_float, C.c_int32, C.c_int32, f32p, C.c_float,
+                                       C.c_float] + [C.c_int32] * 6 + [f32p]
+    L.orc_proj_observe_f32.restype = C.c_int32
+    L.orc_proj_create.restype = C.c_void_p
+    L.orc_proj_create.argtypes = [C.c_float, C.c_float, C.c_int32, C.c_int32]
+    L.orc_proj_destroy.argtypes = [C.c_void_p]
+    L.orc_proj_integrate.argtypes = [C.c_void_p, i64p, C.c_int32, f32p, C.c_int32,
+                                     C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                     f64p, f64p, f32p, f32p]
+    L.orc_proj_integrate.restype = C.c_int64
+    L.orc_proj_block_count.argtypes = [C.c_void_p]
+    L.orc_proj_block_count.restype = C.c_int32
+    L.orc_proj_block.argtypes = [C.c_void_p, C.c_int64, f64p, i32p]
+    L.orc_proj_block.restype = C.c_int32
+    return L
 
 
 # ---- scene helpers --------------------------------------------------------
@@ -243,8 +252,8 @@ class Volume:
 class ProjOracle:
     """FP64 projective-update oracle over touched block lists."""
 
-    def __init__(self, voxel: float, trunc: float, w_max: int, cap_blocks: int):
-        self._L = lib()
+    def __init__(self, voxel: float, trunc: float, w_max: int, cap_blocks: int, L=None):
+        self._L = L if L is not None else lib()
         self.h = self._L.orc_proj_create(voxel, trunc, w_max, cap_blocks)
         if not self.h:
             raise ValueError("orc_proj_create failed")

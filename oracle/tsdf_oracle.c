@@ -44,6 +44,13 @@
 #define ORC_COORD_BIAS ((int64_t)1 << 20)
 #define ORC_COORD_BITS 21
 
+/* ORC_PLANT selects a deliberately wrong running average; only the tests that
+ * show the average's closed-form pin catches such mistakes build it (into a
+ * separate library). 0 is the oracle. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 /* ---- a1/a2: key codec, hash, floor division ------------------------------ */
 
 /* types.hpp:64-69 */
@@ -657,7 +664,12 @@ int32_t orc_backproject_f32(const float* depth, int32_t W, int32_t H, float fx, 
  *   pixel = (floor(fx * p.x / p.z + cx + 0.5), floor(fy * p.y / p.z + cy + 0.5))
  *   skip if outside the image, or if its depth d is not > 0
  *   sdf = d - p.z; skip if sdf < -trunc; x = min(sdf * (1 / trunc), 1)
- *   t <- (t * w + x) / (w + 1);  w <- min(w + 1, w_max)
+ *   skip if w >= w_max (the voxel is frozen at the cap, as the reference's
+ *   weight_cap freezes it: types.hpp:99-101, volume.cpp:283-289)
+ *   t <- (t * w + x) / (w + 1);  w <- w + 1
+ * so after n observations t is the mean of the first min(n, w_max) of them
+ * and w = min(n, w_max): the reference's read-side mean sum / w
+ * (volume.cpp:322-324) of the same observations with unit weights.
  * The discrete decisions (pixel, the z and sdf gates) are evaluated in FP32
  * exactly as the device evaluates them, so that this oracle and the device
  * make the same choices; the continuous part (p, sdf, the running average) is
@@ -806,7 +818,8 @@ void orc_proj_destroy(OrcProj* p) {
     free(p);
 }
 
-/* w_max: weight saturation (weight_cap when > 0, else 65535 for u16 storage). */
+/* w_max: the weight at which a voxel freezes (ceil(weight_cap) when the cap is
+ * > 0, else 65535, the u16 storage's largest weight). */
 OrcProj* orc_proj_create(float voxel, float trunc, int32_t w_max, int32_t cap_blocks) {
     if (!(voxel > 0.0f) || cap_blocks <= 0) return NULL;
     OrcProj* p = (OrcProj*)calloc(1, sizeof(OrcProj));
@@ -881,8 +894,28 @@ int64_t orc_proj_integrate(OrcProj* p, const int64_t* touched, int32_t n_touched
             if (x < -1.0) x = -1.0;
             const size_t o = (size_t)bi * ORC_BLOCK_VOXELS + (size_t)((lz * 8 + ly) * 8 + lx);
             const int32_t w = p->weight[o];
+#if ORC_PLANT == 1  /* planted: no freeze, the cap only bounds the divisor */
             p->tsdf[o] = (p->tsdf[o] * (double)w + x) / (double)(w + 1);
             p->weight[o] = w + 1 < p->w_max ? w + 1 : p->w_max;
+#elif ORC_PLANT == 2  /* planted: cap applied before averaging */
+            const int32_t wc = w + 1 < p->w_max ? w + 1 : p->w_max;
+            p->tsdf[o] = (p->tsdf[o] * (double)w + x) / (double)wc;
+            p->weight[o] = wc;
+#elif ORC_PLANT == 3  /* planted: last observation only */
+            if (w >= p->w_max) continue;
+            p->tsdf[o] = x;
+            p->weight[o] = w + 1;
+#elif ORC_PLANT == 4  /* planted: frozen one observation early */
+            if (w + 1 >= p->w_max) continue;
+            p->tsdf[o] = (p->tsdf[o] * (double)w + x) / (double)(w + 1);
+            p->weight[o] = w + 1;
+#else
+            /* a voxel whose weight has reached the cap is frozen: later
+             * observations do not move it (types.hpp:99-101, volume.cpp:289) */
+            if (w >= p->w_max) continue;
+            p->tsdf[o] = (p->tsdf[o] * (double)w + x) / (double)(w + 1);
+            p->weight[o] = w + 1;
+#endif
             ++n_obs;
         }
     }

@@ -70,7 +70,7 @@ def test_device_view_layout_matches_reference():
     assert offsets == [0, 8, 16, 24, 32, 40, 48, 56, 64, 72, 76, 80, 84, 88, 92, 96]
 
 
-def test_config_and_frame_layout_match_header():
+def test_config_and_frame_layout_match_header(tmp_path):
     """The ctypes mirrors of OsnTsdfHipConfig and OsnTsdfHipFrame have the
     header's field count and size (compiled probe of the C header)."""
     import osn_tsdf_hip as H
@@ -78,10 +78,8 @@ def test_config_and_frame_layout_match_header():
            'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(OsnTsdfHipConfig),'
            ' sizeof(OsnTsdfHipFrame), offsetof(OsnTsdfHipFrame, R_wc64),'
            ' offsetof(OsnTsdfHipFrame, radius_m)); return 0;}\n')
-    tmp = os.path.join(ROOT, "_work")
-    os.makedirs(tmp, exist_ok=True)
-    c = os.path.join(tmp, "layout_probe.c")
-    exe = os.path.join(tmp, "layout_probe")
+    c = str(tmp_path / "layout_probe.c")
+    exe = str(tmp_path / "layout_probe")
     open(c, "w").write(src)
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c], check=True)
     got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True,

@@ -7,6 +7,8 @@ Pins (the projective update has no reference counterpart):
     same back-projected points, bit-exact (allocation is defined on points);
   * touched set per frame == the oracle's distinct visited blocks;
   * FP64 device oracle kernel == CPU FP64 oracle, bit for bit;
+  * FP64, FP32 and FP16 running averages against their closed form (the mean
+    of a voxel's first min(n, cap) observations) on a constant-depth wall;
   * FP32 / FP16 storage against the FP64 oracle: weights exact, max |dtsdf|
     within 1e-4 (FP32) and 0.01 (FP16), in units of the truncation distance
     (FP32 camera depth at 3-6 m carries ~5e-7 m of rounding, ~1.2e-5 trunc
@@ -106,6 +108,40 @@ def test_depth_allocation_key_sets_bit_exact(H, S, trunc):
     assert gv.drop_count == 0
 
 
+def test_scatter_update_on_small_frames_within_1e5(H, S):
+    """The reference-semantics scatter arm on the small workload's 20 frames
+    (640x480, 2 cm voxels, table 2^18), back-projected on the device: key set
+    bit-exact, weights exact and tsdf sums within 1e-5 of sum|terms| (unit
+    weights) against the oracle, at weight_cap 0 (cpp/src/volume.cpp:200-299)."""
+    cfg = dict(S.SMALL)
+    _, fr = frames_for(H, S, cfg, 5, cfg["frames"])
+    pool = cfg["table"] // 2
+    gv = H.HipVolume(cfg["voxel"], pool, cfg["trunc"], 0.0)
+    ov = B.Volume(cfg["voxel"], pool, cfg["trunc"], 0.0)
+    for k, (f, d, R, t) in enumerate(fr):
+        W, Hh = f.width, f.height
+        buf = torch.empty(W * Hh * 3, dtype=torch.float32, device="cuda")
+        assert H.backproject(d, f, buf) == 0
+        p3 = buf.view(-1, 3)
+        dpts = p3[~torch.isnan(p3[:, 0])].contiguous().view(-1)
+        cam = tuple(f.t_wc[:])
+        assert gv.integrate(dpts, cam) == 0
+        ov.integrate(dpts.cpu().numpy(), cam)
+    gv.synchronize()
+    assert gv.drop_count == 0 and gv.block_count == ov.block_count
+    gk = gv.keys()
+    assert np.array_equal(np.sort(gk), np.sort(ov.keys()))
+    t, w = gv.read_voxels()
+    worst = 0.0
+    for i, k in enumerate(gk):
+        ot, ow = ov.block_voxels(int(k))[:2]
+        assert np.array_equal(w[i], ow), f"weights differ in block {k}"
+        e = np.abs(t[i] - ot) / np.maximum(ow, 1.0)
+        worst = max(worst, float(e.max()))
+    print(f"{len(gk)} blocks, max |dsum| / weight {worst:.2e}")
+    assert worst <= 1e-5
+
+
 def _run(H, storage, cfg, fr, wcap):
     v = H.HipVolume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], wcap, storage=storage)
     for f, d, R, t in fr:
@@ -194,6 +230,88 @@ def test_fp32_and_fp16_against_fp64(H, S, wcap):
         print(f"storage {storage}: max {err.max():.3e} rms {np.sqrt((err ** 2).mean()):.3e} "
               f"over {obs.sum()} voxels")
         assert err.max() <= tol
+
+
+def _wall_expectation(keys, touched_per_frame, depths, voxel, trunc, w_max, W, Hh, f):
+    """Closed form of the running average with the cap's freeze, per voxel of
+    every block in keys: t = mean of the first min(m, w_max) observations,
+    w = min(m, w_max), where frame k observes a voxel of a block it touched
+    when the voxel projects inside the image and d_k - z >= -trunc (identity
+    pose, constant depth d_k). Returns (t, w, checkable)."""
+    n = len(keys)
+    t = np.zeros((n, 512))
+    w = np.zeros((n, 512), np.int64)
+    ok = np.ones((n, 512), bool)
+    lz, ly, lx = np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")
+    lx, ly, lz = lx.reshape(512), ly.reshape(512), lz.reshape(512)
+    vs, tr = float(np.float32(voxel)), float(np.float32(trunc))  # the volume's FP32 values
+    for i, k in enumerate(keys):
+        b = [((int(k) >> s) & ((1 << 21) - 1)) - (1 << 20) for s in (42, 21, 0)]
+        x = ((b[0] * 8 + lx) + 0.5) * vs
+        y = ((b[1] * 8 + ly) + 0.5) * vs
+        z = ((b[2] * 8 + lz) + 0.5) * vs
+        u = f * x / z + W / 2 + 0.5
+        v = f * y / z + Hh / 2 + 0.5
+        inside = (u >= 0) & (u < W) & (v >= 0) & (v < Hh)
+        edge = (np.minimum(np.abs(u), np.abs(u - W)) < 1e-3) | \
+               (np.minimum(np.abs(v), np.abs(v - Hh)) < 1e-3)
+        ok[i] &= ~edge
+        s_ = np.zeros(512)
+        c_ = np.zeros(512, np.int64)
+        for d, touched in zip(depths, touched_per_frame):
+            if int(k) not in touched:
+                continue
+            sdf = float(d) - z
+            ok[i] &= ~(np.abs(sdf + tr) < 1e-5)
+            take = inside & (sdf >= -tr) & (c_ < w_max)
+            s_[take] += np.clip(sdf / tr, -1.0, 1.0)[take]
+            c_[take] += 1
+        t[i] = np.where(c_ > 0, s_ / np.maximum(c_, 1), 0.0)
+        w[i] = c_
+    return t, w, ok
+
+
+# The running average against its closed form, with no oracle arithmetic: a
+# camera at the origin looking along +z sees a wall at a different constant
+# depth in each of 12 frames, so every observation x = clamp((d - z) / trunc)
+# is known. With cap 5 most voxels in front of the walls saturate and must
+# freeze (the reference's cap, types.hpp:99-101); with no cap every
+# observation counts. Touched blocks come from the oracle's allocation.
+@pytest.mark.parametrize("wcap", [5.0, 0.0])
+@pytest.mark.parametrize("storage,tol", [("STORE_F64", 1e-12), ("STORE_F32", 2e-5),
+                                         ("STORE_F16", 5e-3)])
+def test_running_average_closed_form_on_wall(H, storage, tol, wcap):
+    W, Hh, f = 64, 48, 32.0
+    voxel, trunc = 0.02, 0.08
+    depths = np.random.default_rng(7).uniform(0.95, 1.35, 12).astype(np.float32)
+    fr = H.make_frame(W, Hh, f, f, W / 2, Hh / 2, np.eye(3), np.zeros(3))
+    v = H.HipVolume(voxel, 1 << 14, trunc, wcap, storage=getattr(H, storage))
+    ov = B.Volume(voxel, 1 << 14, trunc, 0.0)
+    touched = []
+    imgs = [torch.full((Hh * W,), float(d), dtype=torch.float32, device="cuda") for d in depths]
+    for d, img in zip(depths, imgs):
+        assert v.integrate_depth(img, fr) == 0
+        pts = B.backproject_f32(np.full((Hh, W), d, np.float32), f, f, W / 2, Hh / 2,
+                                np.eye(3), np.zeros(3))
+        ov.allocate(pts, (0.0, 0.0, 0.0))
+        touched.append(set(int(k) for k in ov.touched_keys(pts, (0.0, 0.0, 0.0))))
+    v.synchronize()
+    keys = v.keys()
+    assert np.array_equal(np.sort(keys), np.sort(ov.keys()))
+    if storage == "STORE_F64":
+        t, w = v.read_voxels_f64()
+    else:
+        t, w = v.read_voxels()
+    w_max = 5 if wcap > 0 else 65535
+    te, we, ok = _wall_expectation(keys, touched, depths, voxel, trunc, w_max, W, Hh, f)
+    assert ok.sum() > 20000
+    assert np.array_equal(w.astype(np.int64)[ok], we[ok])
+    err = np.abs(t.astype(np.float64) - te)[ok]
+    print(f"{storage} cap {wcap}: max |t - closed form| {err.max():.3e} over {ok.sum()} voxels, "
+          f"{int((we[ok] == w_max).sum())} saturated")
+    assert err.max() <= tol
+    if wcap > 0:
+        assert (we[ok] == w_max).sum() > 5000
 
 
 def test_projective_mesh_on_analytic_surface_and_deterministic(H, S):

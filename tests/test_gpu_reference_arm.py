@@ -51,6 +51,48 @@ def test_sphere_keys_and_sums_bit_parity(H):
         assert np.all(np.abs(t[i] - ot) <= tol), f"tsdf sums differ in block {k}"
 
 
+# Weight caps: the reference skips a visit once the voxel's weight has reached
+# the cap (cpp/src/volume.cpp:283-289, types.hpp:99-101); its sequential CPU
+# arm therefore ends every voxel at min(cap, visits) for an integer cap, and
+# ceil(cap) for a fractional one. The workgroup sums are claimed against the
+# global weight, so the device ends at the same counts whatever the order of
+# its workgroups. Three integrates of the sphere: most surface voxels pass
+# cap 8 within one call, many pass 32.5 by the third.
+@pytest.mark.parametrize("cap", [8.0, 32.5])
+def test_scatter_weight_cap_counts_exact(H, cap):
+    pts = B.sphere_points(0.5, 400, 800)
+    d = dev(pts)
+    gv = H.HipVolume(0.01, 1 << 14, -1.0, cap)
+    oc = B.Volume(0.01, 1 << 14, -1.0, cap)   # sequential, capped
+    ou = B.Volume(0.01, 1 << 14, -1.0, 0.0)   # uncapped: visit counts and sums
+    for _ in range(3):
+        assert gv.integrate(d, (0, 0, 0)) == 0
+        oc.integrate(pts, (0, 0, 0))
+        ou.integrate(pts, (0, 0, 0))
+    gk = gv.keys()
+    assert np.array_equal(np.sort(gk), np.sort(oc.keys()))
+    t, w = gv.read_voxels()
+    wmax = float(np.ceil(cap))
+    n_sat = n_free = 0
+    dev_means = []
+    for i, k in enumerate(gk):
+        cw = oc.block_voxels(int(k))[1]
+        ut, uw = ou.block_voxels(int(k))[:2]
+        assert np.array_equal(w[i], cw), f"capped weights differ in block {k}"
+        free = uw <= wmax  # never capped: the full sums
+        assert np.all(np.abs(t[i][free] - ut[free]) <= 1e-5 * np.maximum(uw[free], 1.0))
+        sat = uw > wmax
+        assert np.all(w[i][sat] == wmax)
+        n_sat += int(sat.sum())
+        n_free += int((free & (uw > 0)).sum())
+        if sat.any():
+            dev_means.append(np.abs(t[i][sat] / wmax))
+    m = np.concatenate(dev_means)
+    print(f"cap {cap}: {n_sat} saturated voxels, {n_free} below the cap; |mean| max {m.max():.3f}")
+    assert n_sat > 10000 and n_free > 1000
+    assert m.max() <= 1.0 + 1e-6  # a convex weighting of observations in [-1, 1]
+
+
 def test_sphere_analytic_gate_and_known_answer(H):
     """cpp/tests/test_cuda_analytic.cu:185-227 with the default cap of 32;
     1,160 blocks and 846,288 vertices are the published counts

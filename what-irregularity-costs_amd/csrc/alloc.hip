@@ -1124,6 +1124,34 @@ __device__ inline void scatter_global(const VolDev& v, size_t idx, float w, floa
     unsafeAtomicAdd(&v.b[idx], c);
 }
 
+// Capped volumes: the workgroup's w visits of a voxel (sum t) are added only
+// as far as the cap allows. The weight is claimed with a compare-and-swap
+// loop, so concurrent flushes of one voxel are serialised and the voxel ends
+// at exactly min(cap, weight before the call + its visits), rounded up to
+// whole visits (a visit is taken while weight < cap, volume.cpp:289): the
+// count the reference's sequential CPU arm reaches. When only part of the w
+// visits fit, their sum is taken in proportion (t * take / w), a convex
+// weighting of the same observations; which visits the reference keeps at
+// the cap depends on its thread timing anyway.
+__device__ inline void scatter_global_capped(const VolDev& v, size_t idx, float w, float t) {
+    unsigned int* wp = reinterpret_cast<unsigned int*>(&v.weight[idx]);
+    unsigned int old = __atomic_load_n(wp, __ATOMIC_RELAXED);
+    float take;
+    for (;;) {
+        const float cur = __uint_as_float(old);
+        if (!(cur < v.weight_cap)) return;
+        take = fminf(w, ceilf(v.weight_cap - cur));
+        const unsigned int seen = atomicCAS(wp, old, __float_as_uint(cur + take));
+        if (seen == old) break;
+        old = seen;
+    }
+    unsafeAtomicAdd(&v.tsdf[idx], take == w ? t : t * (take / w));
+    const float c = 128.0f * take;
+    unsafeAtomicAdd(&v.r[idx], c);
+    unsafeAtomicAdd(&v.g[idx], c);
+    unsafeAtomicAdd(&v.b[idx], c);
+}
+
 __global__ __launch_bounds__(256) void k_scatter_update(VolDev v, PointSrc src) {
     __shared__ ScatterLds s;
     for (int e = threadIdx.x; e < kScatterSet; e += blockDim.x) {
@@ -1169,26 +1197,62 @@ __global__ __launch_bounds__(256) void k_scatter_update(VolDev v, PointSrc src) 
             const size_t idx = (size_t)last_bi * kBlockVoxels +
                                (size_t)(((vz & 7) * 8 + (vy & 7)) * 8 + (vx & 7));
             // the reference's racy cap check, against the global weight
-            if (v.weight_cap > 0.0f && glob(v.weight)[idx] >= v.weight_cap) continue;
+            // (volume.cpp:283-289)
+            const bool capped = v.weight_cap > 0.0f;
+            const float g = capped ? glob(v.weight)[idx] : 0.0f;
+            if (capped && g >= v.weight_cap) continue;
             uint32_t h = uint32_t((uint64_t(idx) * 0x9E3779B97F4A7C15ull) >> 40) & (kScatterSet - 1);
             bool placed = false;
             for (int p = 0; p < 16; ++p) {
                 const unsigned long long prev = atomicCAS(&s.key[h], ~0ull, (unsigned long long)idx);
                 if (prev == ~0ull || prev == (unsigned long long)idx) {
-                    atomicAdd(&s.w[h], 1.0f);
-                    atomicAdd(&s.t[h], sdf_n);
+                    if (capped) {
+                        // The workgroup's own visits are counted against the
+                        // cap too: a visit claims one unit of the LDS weight
+                        // only while the global weight it read plus the
+                        // workgroup's claims so far stay below the cap, so a
+                        // workgroup carries at most the visits the cap still
+                        // admits to the flush, which then claims them from
+                        // the global weight (scatter_global_capped).
+                        unsigned int* wb = reinterpret_cast<unsigned int*>(&s.w[h]);
+                        unsigned int old = __atomic_load_n(wb, __ATOMIC_RELAXED);
+                        bool claimed = false;
+                        while (g + __uint_as_float(old) < v.weight_cap) {
+                            const unsigned int seen =
+                                atomicCAS(wb, old, __float_as_uint(__uint_as_float(old) + 1.0f));
+                            if (seen == old) {
+                                claimed = true;
+                                break;
+                            }
+                            old = seen;
+                        }
+                        if (claimed) atomicAdd(&s.t[h], sdf_n);
+                    } else {
+                        atomicAdd(&s.w[h], 1.0f);
+                        atomicAdd(&s.t[h], sdf_n);
+                    }
                     placed = true;
                     break;
                 }
                 h = (h + 1) & (kScatterSet - 1);
             }
-            if (!placed) scatter_global(v, idx, 1.0f, sdf_n);  // LDS table crowded
+            if (!placed) {  // LDS table crowded
+                if (capped)
+                    scatter_global_capped(v, idx, 1.0f, sdf_n);
+                else
+                    scatter_global(v, idx, 1.0f, sdf_n);
+            }
         }
     }
     __syncthreads();
+    const bool capped = v.weight_cap > 0.0f;
     for (int e = threadIdx.x; e < kScatterSet; e += blockDim.x) {
         const unsigned long long k = s.key[e];
-        if (k != ~0ull) scatter_global(v, (size_t)k, s.w[e], s.t[e]);
+        if (k == ~0ull) continue;
+        if (!capped)
+            scatter_global(v, (size_t)k, s.w[e], s.t[e]);
+        else if (s.w[e] > 0.0f)  // a slot can hold a voxel none of whose visits fit
+            scatter_global_capped(v, (size_t)k, s.w[e], s.t[e]);
     }
 }
 

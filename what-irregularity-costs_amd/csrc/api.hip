@@ -297,8 +297,13 @@ OsnTsdfHipVolume* make_volume(const OsnTsdfHipConfig& c, bool reference_shape) {
     d.weight_cap = c.weight_cap;
     const int32_t cs = (int32_t)std::ceil(trunc * d.inv_voxel);  // volume.cpp:171
     d.steps = cs > 1 ? cs : 1;
-    d.w_max = c.weight_cap > 0.0f ? (int32_t)(c.weight_cap < 65535.0f ? c.weight_cap : 65535.0f)
-                                  : 65535;
+    // projective storage: a voxel freezes once its weight reaches the cap. The
+    // reference skips an observation when weight >= cap (volume.cpp:289), so
+    // unit weights stop at ceil(cap); no cap means the u16 weight's largest
+    // value.
+    d.w_max = c.weight_cap > 0.0f
+                  ? (int32_t)std::ceil(c.weight_cap < 65535.0f ? c.weight_cap : 65535.0f)
+                  : 65535;
     if (d.w_max < 1) d.w_max = 1;
     v->lds_set = c.lds_set < 0 ? 1024 : c.lds_set;
     v->update_threads = c.update_threads > 0 ? c.update_threads : 256;

@@ -114,7 +114,7 @@ struct VolDev {
     float trunc;
     float weight_cap;
     int32_t steps;         // band half-width in samples
-    int32_t w_max;         // projective weight saturation
+    int32_t w_max;         // projective: weight at which a voxel freezes, ceil(cap) or 65535
     // per-call work lists, capacity entries each; new_keys entries carry the
     // slot the key was inserted at (bi unused), sort_a/sort_b are the merge
     // buffers of the index assignment

@@ -137,7 +137,7 @@ hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, hipStream
 hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call, int32_t grid,
                               int32_t block_threads, hipStream_t s);
 // max_w_max: the largest weight cap of the batch's volumes (FP16 records
-// with caps below 64 take their averaging reciprocals from a table)
+// with caps up to 64 take their averaging reciprocals from a table)
 hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t grid,
                                     int32_t block_threads, hipStream_t s,
                                     const LaunchEvents& ev = {}, int32_t max_w_max = 65535);

@@ -13,7 +13,13 @@
 //                                       8x4-pixel tiles, one cache line each)
 //   sdf = d - p.z                       (skip if sdf < -trunc)
 //   x   = min(sdf * (1 / trunc), 1)
-//   t  <- (t * w + x) / (w + 1),  w <- min(w + 1, w_max)
+//   skip if w >= w_max                  (frozen at the weight cap, as the
+//                                        reference's cap freezes a voxel:
+//                                        cpp/include/osn_tsdf/types.hpp:99-101)
+//   t  <- (t * w + x) / (w + 1),  w <- w + 1
+//
+// so t is the mean of the voxel's first min(n, w_max) observations, the
+// reference's sum / w (cpp/src/volume.cpp:322-324) with unit weights.
 //
 // FP32 evaluation. The pixel is a ratio of two linear forms of the voxel
 // index, A = fx p.x + (cx + 0.5) p.z and B = fy p.y + (cy + 0.5) p.z, over
@@ -52,7 +58,6 @@ namespace {
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 template <typename T>
 __device__ inline T OSN_GLOBAL* glob(T* p) {
@@ -120,9 +125,9 @@ __device__ inline void run_reciprocals(const float Z[4], float iz[4], float z[4]
     iz[0] = r1 * z[1];
 }
 
-// Weight caps below this many frames: the reciprocals 1 / (w + 1) of every
-// weight a record can hold come from a table, correctly rounded, instead of
-// the transcendental unit.
+// Weight caps up to this many frames: the reciprocals 1 / (w + 1) of every
+// weight a record is updated from (w < cap) come from a table, correctly
+// rounded, instead of the transcendental unit.
 constexpr int kRcpTable = 64;
 
 __device__ inline float avg_step(float t, float w, float x) {
@@ -275,47 +280,51 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
         const bool* o = ok + r * 4;
         const float* sd = sdf + r * 4;
         if constexpr (STORE == kStoreF16) {
-            if (o[0] | o[1] | o[2] | o[3]) {
-                uint32_t word[4] = {R.h[r].x, R.h[r].y, R.h[r].z, R.h[r].w};
-                const unsigned short wcap = (unsigned short)k.wcap;
+            uint32_t word[4] = {R.h[r].x, R.h[r].y, R.h[r].z, R.h[r].w};
+            // a voxel whose weight has reached the cap is frozen
+            bool up[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) up[j] = o[j] & ((word[j] >> 16) < (uint32_t)k.wcap);
+            if (up[0] | up[1] | up[2] | up[3]) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const float x = fminf(sd[j] * k.inv_trunc, 1.0f);
                     const float tt = __half2float(__ushort_as_half((unsigned short)(word[j] & 0xffffu)));
-                    const float w = (float)(word[j] >> 16);
+                    const uint32_t wi = word[j] >> 16;
                     float tn;
                     if constexpr (TAB) {
-                        const uint32_t wi = word[j] >> 16;
-                        tn = __builtin_fmaf(tt, w, x) * rtab[wi < kRcpTable - 1 ? wi : kRcpTable - 1];
+                        // wi < w_max <= kRcpTable when the record is updated
+                        tn = __builtin_fmaf(tt, (float)wi, x) *
+                             rtab[wi < kRcpTable - 1 ? wi : kRcpTable - 1];
                     } else {
-                        tn = avg_step(tt, w, x);
+                        tn = avg_step(tt, (float)wi, x);
                     }
-                    // the weight min(w + 1, w_max) in the high half (saturating
-                    // 16-bit add, then the cap), the new mean in the low half
-                    u16x2 pw = __builtin_bit_cast(u16x2, word[j]);
-                    pw = __builtin_elementwise_add_sat(pw, u16x2{0, 1});
-                    pw = __builtin_elementwise_min(pw, u16x2{0xffff, wcap});
-                    pw.x = __half_as_ushort(__float2half_rn(tn));
-                    word[j] = o[j] ? __builtin_bit_cast(uint32_t, pw) : word[j];
+                    // the weight w + 1 (below the cap, so no carry out of the
+                    // high half), the new mean in the low half
+                    const uint32_t nw = ((word[j] + 0x10000u) & 0xffff0000u) |
+                                        (uint32_t)__half_as_ushort(__float2half_rn(tn));
+                    word[j] = up[j] ? nw : word[j];
                 }
                 u32x4 OSN_GLOBAL* rec = reinterpret_cast<u32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 4);
                 rec[r * 64] = u32x4{word[0], word[1], word[2], word[3]};
             }
         } else if constexpr (STORE == kStoreF32) {
-            if (o[0] | o[1] | o[2] | o[3]) {
-                float tv[4] = {R.h[r][0].x, R.h[r][0].z, R.h[r][1].x, R.h[r][1].z};
-                float wv[4] = {R.h[r][0].y, R.h[r][0].w, R.h[r][1].y, R.h[r][1].w};
+            float tv[4] = {R.h[r][0].x, R.h[r][0].z, R.h[r][1].x, R.h[r][1].z};
+            float wv[4] = {R.h[r][0].y, R.h[r][0].w, R.h[r][1].y, R.h[r][1].w};
+            bool up[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) up[j] = o[j] & (wv[j] < k.wmax);
+            if (up[0] | up[1] | up[2] | up[3]) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const float x = fminf(sd[j] * k.inv_trunc, 1.0f);
                     const float tn = avg_step(tv[j], wv[j], x);
-                    const float wn = fminf(wv[j] + 1.0f, k.wmax);
-                    tv[j] = o[j] ? tn : tv[j];
-                    wv[j] = o[j] ? wn : wv[j];
+                    tv[j] = up[j] ? tn : tv[j];
+                    wv[j] = up[j] ? wv[j] + 1.0f : wv[j];
                 }
                 f32x4 OSN_GLOBAL* rec = reinterpret_cast<f32x4 OSN_GLOBAL*>(glob(k.vox) + v0 * 8);
-                if (o[0] | o[1]) rec[r * 128 + 0] = f32x4{tv[0], wv[0], tv[1], wv[1]};
-                if (o[2] | o[3]) rec[r * 128 + 1] = f32x4{tv[2], wv[2], tv[3], wv[3]};
+                if (up[0] | up[1]) rec[r * 128 + 0] = f32x4{tv[0], wv[0], tv[1], wv[1]};
+                if (up[2] | up[3]) rec[r * 128 + 1] = f32x4{tv[2], wv[2], tv[3], wv[3]};
             }
         } else {
             const int ly = L.ly, lz = L.lz0 + 4 * r;
@@ -325,6 +334,7 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
                 f64x2 OSN_GLOBAL* rec = reinterpret_cast<f64x2 OSN_GLOBAL*>(
                     glob(k.vox) + (v0 + r * 256 + j) * 16);
                 f64x2 q = *rec;
+                if (!(q.y < (double)k.wmax)) continue;  // frozen at the cap
                 const double vs = (double)k.voxel;
                 const double wx = ((double)(bx * 8 + L.lx0 + j) + 0.5) * vs - k.c64[0];
                 const double wy = ((double)(by * 8 + ly) + 0.5) * vs - k.c64[1];
@@ -334,7 +344,7 @@ __device__ inline void update_block(const Params& k, const LaneRuns& L, const Bl
                 x = x > 1.0 ? 1.0 : (x < -1.0 ? -1.0 : x);
                 const double w = q.y;
                 q.x = (q.x * w + x) / (w + 1.0);
-                q.y = (w + 1.0 < (double)k.wmax) ? w + 1.0 : (double)k.wmax;
+                q.y = w + 1.0;
                 *rec = q;
             }
         }
@@ -470,7 +480,7 @@ hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call
         return hipErrorInvalidValue;
     switch (v.storage) {
         case kStoreF16:
-            if (v.w_max < kRcpTable)
+            if (v.w_max <= kRcpTable)
                 k_proj_update<kStoreF16, true><<<grid, block_threads, 0, s>>>(v, cam, call);
             else
                 k_proj_update<kStoreF16, false><<<grid, block_threads, 0, s>>>(v, cam, call);
@@ -492,7 +502,7 @@ hipError_t launch_proj_update_batch(const BatchDev& b, int32_t storage, int32_t 
         return hipErrorInvalidValue;
     switch (storage) {
         case kStoreF16:
-            if (max_w_max < kRcpTable)
+            if (max_w_max <= kRcpTable)
                 launch_kernel(k_proj_update_batch<kStoreF16, true>, dim3(grid), dim3(block_threads), s, ev, b);
             else
                 launch_kernel(k_proj_update_batch<kStoreF16, false>, dim3(grid), dim3(block_threads), s, ev, b);
