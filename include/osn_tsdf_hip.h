@@ -272,6 +272,16 @@ int32_t osn_tsdf_hip_batch_set_extract(OsnTsdfHipBatch* b, float* const* d_posno
                                        const int32_t* vert_caps, int32_t* d_counts,
                                        float min_weight, float iso, int32_t every);
 
+/* Instrumented batched allocation, the batched form of the reference's
+ * measurement variants (cpp/src/volume_cuda.cu:59-61,81 counting CAS,
+ * :109-154,271-300 probe depth): mode 0 normal, 1 counts every compare-and-
+ * swap attempt (into cas_attempts and, as the reference does, drop_count),
+ * 2 records each member volume's probe-depth histogram, lane and wave step
+ * totals and distinct-key lookups (read with osn_tsdf_hip_probe_stats; the
+ * counters accumulate). Results are unchanged. Invalidates a captured graph.
+ * 0 ok, 1 bad arguments, 2 device error. */
+int32_t osn_tsdf_hip_batch_set_alloc_mode(OsnTsdfHipBatch* b, int32_t mode);
+
 /* Stage timing for batched steps, as osn_tsdf_hip_read_timing (stages 0-2,
  * and 3 for the extraction after a step; with use_graph the whole step is
  * reported as stage 0). */

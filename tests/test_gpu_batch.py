@@ -261,3 +261,35 @@ def test_batched_extraction_equals_per_scene(H, S, use_graph):
             del ref
         counts.fill_(-7)
     b.close()
+
+
+# The instrumented allocate modes of a batch (probe-depth statistics, counted
+# compare-and-swap attempts) change no result, and their counters are
+# consistent: every distinct-key lookup lands in one histogram bin, the
+# deepest lane of each 64-lane batch bounds the wave steps from above, and
+# every inserted block took at least one exchange attempt.
+@pytest.mark.parametrize("mode", [1, 2])
+def test_instrumented_batch_allocate(H, S, mode):
+    import ctypes as C
+    n_steps = 5
+    depths, frames = make_inputs(H, S, n_steps, seed0=90)
+    plain = volumes(H, 2)
+    inst = volumes(H, 2)
+    bp, bi = H.Batch(plain), H.Batch(inst)
+    for b in (bp, bi):
+        assert b.set_schedule(depths, frames) == 0
+    assert bi.set_alloc_mode(mode) == 0
+    assert bp.run(n_steps) == 0 and bi.run(n_steps, use_graph=True) == 0
+    bp.synchronize()
+    bi.synchronize()
+    for v1, v2 in zip(plain, inst):
+        assert same(state(v1), state(v2))
+        hist, lane, wave, look = v2.probe_stats()
+        if mode == 2:
+            assert look > v2.block_count and hist.sum() == look
+            assert lane >= look and wave >= lane
+            assert H.cas_attempts(v2) == 0
+        else:
+            assert H.cas_attempts(v2) >= v2.block_count > 0
+            assert look == 0
+    assert bi.set_alloc_mode(3) == 1

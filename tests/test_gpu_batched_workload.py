@@ -1,0 +1,170 @@
+"""The batched workload on the GPU: 24 independent seeded scenes of uneven
+size (320x240, 640x480 or 1280x720 depth, frame counts U{50..400} scaled by
+0.1 to at most 40 steps, tables 2^16..2^22 slots), FP16 storage, integrated
+together in one schedule replayed from a captured graph.
+
+Pins:
+  * every scene's batched state (pool order, keys, records, block and drop
+    counts) is byte-identical to integrating that scene alone through the
+    single-volume calls: no two scenes share memory and the arithmetic is the
+    same, whatever the mix of sizes, table capacities and idle steps;
+  * the first step's allocated key set of every scene equals the oracle's
+    allocation over the same back-projected points (cpp/src/volume.cpp:147-198);
+  * FP16 storage stays within 0.01 trunc of the FP64 oracle kernel on two
+    scenes of different image size, with exact weights.
+Scenes idle (NULL depth) after their last frame, and a fifth of them also
+idle every fourth step before it.
+"""
+import numpy as np
+import pytest
+
+import bindings as B
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+SEED = 4
+SCALE = 0.1
+
+
+@pytest.fixture(scope="module")
+def H():
+    import osn_tsdf_hip
+    assert torch.cuda.is_available()
+    return osn_tsdf_hip
+
+
+@pytest.fixture(scope="module")
+def S():
+    import scenes
+    return scenes
+
+
+def idle(s, k):
+    return s % 5 == 3 and k % 4 == 2
+
+
+@pytest.fixture(scope="module")
+def workload(H, S):
+    cfgs = S.batched_workload(SEED, 24, SCALE)
+    n_steps = max(c["frames"] for c in cfgs)
+    n = len(cfgs)
+    depths = [[None] * n for _ in range(n_steps)]
+    frames = [[None] * n for _ in range(n_steps)]
+    for s, c in enumerate(cfgs):
+        scene, poses = S.build(c, c["seed"])
+        W, Hh = c["width"], c["height"]
+        fx, fy, cx, cy = S.intrinsics(W, Hh)
+        sc = scene.to_c()
+        for k, (R, t) in enumerate(poses):
+            if idle(s, k):
+                continue
+            fr = H.make_frame(W, Hh, fx, fy, cx, cy, R, t)
+            d = torch.empty(W * Hh, dtype=torch.float32, device="cuda")
+            assert S.render_depth(sc, fr, c["noise_k"], c["dropout"], c["seed"], k, d) == 0
+            depths[k][s] = d
+            frames[k][s] = fr
+    torch.cuda.synchronize()
+    print("steps", n_steps, "sizes", sorted(set((c["width"], c["height"]) for c in cfgs)),
+          "tables", sorted(set(c["table"].bit_length() - 1 for c in cfgs)))
+    return cfgs, depths, frames, n_steps
+
+
+def volume(H, c, storage):
+    return H.HipVolume(c["voxel"], c["table"] // 2, c["trunc"], 32.0, storage=storage,
+                       table_slots=c["table"])
+
+
+def state(v):
+    k = v.keys()
+    t, w = v.read_voxels()
+    return k, t.view(np.uint32), w.view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def batched(H, workload):
+    """The batch after its first step (key sets kept) and after every step."""
+    cfgs, depths, frames, n_steps = workload
+    vols = [volume(H, c, H.STORE_F16) for c in cfgs]
+    b = H.Batch(vols)
+    assert b.set_schedule(depths, frames) == 0
+    assert b.run(1, use_graph=True) == 0
+    b.synchronize()
+    first = [(v.keys(), v.drop_count) for v in vols]
+    assert b.run(n_steps - 1, use_graph=True) == 0
+    b.synchronize()
+    b.close()
+    return vols, first
+
+
+def test_first_step_key_sets_match_oracle(H, workload, batched):
+    cfgs, depths, frames, _ = workload
+    _, first = batched
+    checked = 0
+    for s, c in enumerate(cfgs):
+        d, fr = depths[0][s], frames[0][s]
+        if d is None:
+            continue
+        W, Hh = fr.width, fr.height
+        buf = torch.empty(W * Hh * 3, dtype=torch.float32, device="cuda")
+        assert H.backproject(d, fr, buf) == 0
+        p = buf.cpu().numpy().reshape(-1, 3)
+        pts = np.ascontiguousarray(p[~np.isnan(p[:, 0])].ravel())
+        ov = B.Volume(c["voxel"], c["table"] // 2, c["trunc"], 0.0)
+        ov.allocate(pts, tuple(fr.t_wc[:]))
+        keys, drops = first[s]
+        if drops or ov.drop_count:
+            continue  # a full pool drops different keys by design (largest keys first)
+        assert np.array_equal(np.sort(keys), np.sort(ov.keys())), f"scene {s}"
+        checked += 1
+    assert checked >= 20
+
+
+def test_batched_state_equals_per_scene(H, workload, batched):
+    cfgs, depths, frames, n_steps = workload
+    vols, _ = batched
+    drops = 0
+    for s, c in enumerate(cfgs):
+        solo = volume(H, c, H.STORE_F16)
+        for k in range(n_steps):
+            if depths[k][s] is not None:
+                assert solo.integrate_depth(depths[k][s], frames[k][s]) == 0
+        solo.synchronize()
+        v = vols[s]
+        assert v.block_count == solo.block_count > 0, f"scene {s}"
+        assert v.drop_count == solo.drop_count, f"scene {s}"
+        a, b = state(v), state(solo)
+        assert all(np.array_equal(x, y) for x, y in zip(a, b)), f"scene {s}"
+        drops += int(v.drop_count > 0)
+        solo.close()
+    print("scenes whose pool filled:", drops)
+
+
+def test_fp16_within_gate_on_two_scenes(H, workload, batched):
+    cfgs, depths, frames, n_steps = workload
+    vols, _ = batched
+    # the scene with the most frames of the largest and of the smallest size
+    pick = []
+    for size in ((1280, 720), (320, 240)):
+        cand = [s for s, c in enumerate(cfgs) if (c["width"], c["height"]) == size]
+        pick.append(max(cand, key=lambda s: cfgs[s]["frames"]))
+    for s in pick:
+        c = cfgs[s]
+        ref = volume(H, c, H.STORE_F64)
+        for k in range(n_steps):
+            if depths[k][s] is not None:
+                assert ref.integrate_depth(depths[k][s], frames[k][s]) == 0
+        ref.synchronize()
+        k64 = ref.keys()
+        t64, w64 = ref.read_voxels_f64()
+        k16 = vols[s].keys()
+        t16, w16 = vols[s].read_voxels()
+        assert np.array_equal(k16, k64)
+        assert np.array_equal(w16.astype(np.float64), w64)
+        obs = w64 > 0
+        err = np.abs(t16.astype(np.float64) - t64)[obs]
+        print(f"scene {s} {c['width']}x{c['height']}, {c['frames']} frames: max |dt| "
+              f"{err.max():.2e} trunc, rms {np.sqrt((err ** 2).mean()):.2e}, {obs.sum()} voxels")
+        assert obs.sum() > 100000
+        assert err.max() <= 0.01
+        ref.close()

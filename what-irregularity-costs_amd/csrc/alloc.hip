@@ -722,6 +722,7 @@ __global__ __launch_bounds__(kAllocThreads, kAllocWavesPerSimd) void k_alloc(Vol
 // one work list (scene s owns [tile_prefix[s], tile_prefix[s + 1]), read once
 // into LDS); each scene keeps its own table and lists, flushed when the
 // workgroup moves on to another scene.
+template <int MODE>
 __global__ __launch_bounds__(kAllocThreads, kAllocWavesPerSimd) void k_alloc_batch(BatchDev b, int32_t set_size,
                                                                int32_t total_tiles) {
     __shared__ WgShared s;
@@ -745,7 +746,7 @@ __global__ __launch_bounds__(kAllocThreads, kAllocWavesPerSimd) void k_alloc_bat
         if (sc != cur) {
             if (cur >= 0) {
                 flush_lists(v, &v.ctr->lists[call.parity], s);
-                flush_stats<kAllocNormal>(v, s);
+                flush_stats<MODE>(v, s);
             }
             cur = sc;
             v = load_const(b.vols + sc);
@@ -754,11 +755,11 @@ __global__ __launch_bounds__(kAllocThreads, kAllocWavesPerSimd) void k_alloc_bat
         }
         clear_set(s, set_size);
         const TileCtx c{&v, &v.ctr->lists[call.parity], &s, set_size, call.stamp};
-        alloc_depth_unit<kAllocNormal, true>(c, src, local, kBatchDepthSub);
+        alloc_depth_unit<MODE, true>(c, src, local, kBatchDepthSub);
     }
     if (cur >= 0) {
         flush_lists(v, &v.ctr->lists[call.parity], s);
-        flush_stats<kAllocNormal>(v, s);
+        flush_stats<MODE>(v, s);
     }
 }
 
@@ -1326,7 +1327,21 @@ hipError_t launch_alloc_batch(const BatchDev& b, int32_t total_tiles, const Allo
     const int set = clamp_set(o.lds_set);
     if (set & (set - 1)) return hipErrorInvalidValue;
     const int grid = o.grid > 0 ? min(total_tiles, o.grid) : total_tiles;
-    launch_kernel(k_alloc_batch, dim3(grid), dim3(kAllocThreads), s, ev, b, set, total_tiles);
+    switch (o.mode) {
+        case kAllocNormal:
+            launch_kernel(k_alloc_batch<kAllocNormal>, dim3(grid), dim3(kAllocThreads), s, ev, b, set,
+                          total_tiles);
+            break;
+        case kAllocCountCas:
+            launch_kernel(k_alloc_batch<kAllocCountCas>, dim3(grid), dim3(kAllocThreads), s, ev, b,
+                          set, total_tiles);
+            break;
+        case kAllocProbe:
+            launch_kernel(k_alloc_batch<kAllocProbe>, dim3(grid), dim3(kAllocThreads), s, ev, b, set,
+                          total_tiles);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -883,6 +883,7 @@ struct OsnTsdfHipBatch {
     int32_t max_w_max = 0;  // the largest projective weight cap of the volumes
     std::vector<int64_t> base_calls;
     int32_t alloc_grid = 0, update_grid = 0, update_threads = 256, lds_set = 1024;
+    int32_t alloc_mode = kAllocNormal;  // instrumentation of the allocate kernel
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // mesh extraction every `extract_every` steps (0: never), per scene into
@@ -970,6 +971,7 @@ struct OsnTsdfHipBatch {
         AllocOpts o;
         o.lds_set = lds_set;
         o.grid = alloc_grid;
+        o.mode = alloc_mode;
         // With timing on, each stage's kernel records its own start and end
         // (stage i: events 2i, 2i + 1).
         hipEvent_t e[6] = {};
@@ -1150,6 +1152,14 @@ int32_t osn_tsdf_hip_batch_tune(OsnTsdfHipBatch* b, int32_t alloc_grid, int32_t 
     if (update_grid >= 0) b->update_grid = update_grid;
     if (update_threads) b->update_threads = update_threads;
     if (lds_set >= 0) b->lds_set = lds_set;
+    return 0;
+}
+
+int32_t osn_tsdf_hip_batch_set_alloc_mode(OsnTsdfHipBatch* b, int32_t mode) {
+    if (!b || mode < 0 || mode > 2) return 1;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return 2;
+    b->drop_graph();
+    b->alloc_mode = mode == 1 ? kAllocCountCas : (mode == 2 ? kAllocProbe : kAllocNormal);
     return 0;
 }
 

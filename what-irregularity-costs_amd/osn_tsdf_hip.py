@@ -125,6 +125,7 @@ def lib():
         "osn_tsdf_hip_batch_set_timing": (_I, [_VP, _I]),
         "osn_tsdf_hip_batch_read_timing": (_I, [_VP, _VP, _VP]),
         "osn_tsdf_hip_batch_set_extract": (_I, [_VP, _VP, _VP, _VP, _F, _F, _I]),
+        "osn_tsdf_hip_batch_set_alloc_mode": (_I, [_VP, _I]),
         "osn_tsdf_hip_insert_blocks_mode": (_I, [_VP, _VP, _I, _I]),
         "osn_tsdf_hip_cas_attempts": (_I, [_VP, _VP]),
         "osn_tsdf_hip_cas_ceiling": (_I, [_I, _I, _VP]),
@@ -376,6 +377,10 @@ class Batch:
 
     def set_timing(self, on: bool):
         self._L.osn_tsdf_hip_batch_set_timing(self.h, 1 if on else 0)
+
+    def set_alloc_mode(self, mode: int) -> int:
+        """0 normal, 1 counting CAS attempts, 2 probe-depth statistics."""
+        return self._L.osn_tsdf_hip_batch_set_alloc_mode(self.h, int(mode))
 
     def read_timing(self):
         ms = np.zeros(4, np.float64)

@@ -12,7 +12,8 @@ Configurations (sizes as the workload descriptions state them):
              640x480, 1 cm voxels, trunc 4 cm, table 2^22, 500 frames, depth
              noise N(0, 0.0012 z^2) and 1% dropouts;
   large   -- 12x12x4 m room, 60 mixed primitives, 1280x960, 5 mm voxels,
-             trunc 2 cm, table 2^25, 1000 frames, same noise model.
+             trunc 2 cm, table 2^25, 1000 frames, same noise model;
+  batched -- 24 independent scenes of uneven size (batched_workload).
 Intrinsics: fx = fy = W/2, cx = W/2, cy = H/2 (fx = 320 at 640x480).
 The camera moves 0.02 m and turns 1 degree of yaw per frame along a circle
 inside the room, with a slow +-10 degree pitch oscillation.
@@ -125,6 +126,33 @@ PRECISE = dict(room=(6.0, 6.0, 3.0), n_spheres=20, n_boxes=10, width=640, height
                voxel=0.01, trunc=0.04, table=1 << 22, frames=500, noise_k=0.0012, dropout=0.01)
 LARGE = dict(room=(12.0, 12.0, 4.0), n_spheres=30, n_boxes=30, width=1280, height=960,
              voxel=0.005, trunc=0.02, table=1 << 25, frames=1000, noise_k=0.0012, dropout=0.01)
+
+
+BATCH_SIZES = ((320, 240), (640, 480), (1280, 720))
+
+
+def batched_workload(seed: int, n_scenes: int = 24, frame_scale: float = 1.0) -> list:
+    """The batched workload: independent scenes of uneven size. Per scene, an
+    image size drawn from 320x240 / 640x480 / 1280x720 (intrinsics as above), a
+    frame count U{50..400} (times frame_scale, at least 1), a table capacity
+    2^U{16..22} slots (pool half of it), a room with sides U[4, 6] m and height
+    3 m (between the small and precise rooms), U{8..20} spheres and U{0..10}
+    boxes; 1 cm voxels, trunc 4 cm, the precise noise model. Returns one
+    configuration dict per scene, with its scene seed and its unscaled
+    frame count."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_scenes):
+        W, H = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        frames = int(rng.integers(50, 401))
+        table = 1 << int(rng.integers(16, 23))
+        side = float(rng.uniform(4.0, 6.0))
+        out.append(dict(room=(side, side, 3.0), n_spheres=int(rng.integers(8, 21)),
+                        n_boxes=int(rng.integers(0, 11)), width=W, height=H, voxel=0.01,
+                        trunc=0.04, table=table, frames=max(1, int(round(frames * frame_scale))),
+                        frames_full=frames, noise_k=0.0012, dropout=0.01,
+                        seed=int(seed) * 1000 + s))
+    return out
 
 
 def build(cfg: dict, seed: int):
