@@ -51,7 +51,12 @@ def _trial(H, make_volumes, depths, frames, warmup, steps, knobs):
     return {"allocate_ms": per[0], "assign_ms": per[1], "update_ms": per[2]}
 
 
-def tune(H, make_volumes, depths, frames, warmup=10, steps=20, log_path=None):
+# one scene at a time has a few thousand blocks per frame: smaller grids
+PLAIN_UPDATE_GRIDS = (512, 1024, 2048, 4096)
+
+
+def tune(H, make_volumes, depths, frames, warmup=10, steps=20, log_path=None,
+         update_grids=UPDATE_GRIDS, alloc_grids=ALLOC_GRIDS):
     """make_volumes(hash_kind) -> fresh volumes for one trial; depths/frames as
     for Batch.set_schedule, at least warmup + steps long. Returns the chosen
     knobs (a dict usable with Batch.tune and the volume config)."""
@@ -66,7 +71,7 @@ def tune(H, make_volumes, depths, frames, warmup=10, steps=20, log_path=None):
 
     # allocate knobs (the update does not depend on them)
     scores = []
-    for g in ALLOC_GRIDS:
+    for g in alloc_grids:
         for s in LDS_SETS:
             for h in HASHES:
                 k = dict(best, alloc_grid=g, lds_set=s, hash_kind=h)
@@ -74,7 +79,7 @@ def tune(H, make_volumes, depths, frames, warmup=10, steps=20, log_path=None):
     best = min(scores, key=lambda x: x[0])[1]
     # update knobs
     shapes = [dict(best, update_threads=t, update_grid=g)
-              for t in UPDATE_THREADS for g in UPDATE_GRIDS]
+              for t in UPDATE_THREADS for g in update_grids]
     sums = [0.0] * len(shapes)
     for _ in range(UPDATE_ROUNDS):
         for i, k in enumerate(shapes):
