@@ -143,7 +143,7 @@ typedef struct {
     uint32_t table_slots;   /* 0: next_pow2(2 * pool) */
     int32_t hash_kind;      /* OSN_TSDF_HIP_HASH_*; REFERENCE needs a power of two */
     uint32_t probe_limit;   /* slots a lane inspects before giving up; 0: table_slots */
-    int32_t lds_set;        /* per-workgroup pre-hash entries: power of two <= 2048, 0 off, -1 default */
+    int32_t lds_set;        /* per-workgroup LDS key set: power of two <= 1024 (larger is clamped), 0 off, -1 default */
     int32_t update_threads; /* projective update workgroup size (128/256/512), 0 default */
     int32_t update_grid;    /* projective update workgroups, 0 default */
     int32_t alloc_grid;     /* allocate workgroups (each loops over pixel tiles), 0 default */

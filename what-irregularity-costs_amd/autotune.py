@@ -2,7 +2,7 @@
 
 Searches, on the caller's real workload (volumes, frames, step count), the
 knobs that change speed but not results:
-  * allocate: workgroups (each loops over 16x16 pixel tiles), LDS pre-hash
+  * allocate: workgroups (each loops over 16x16 pixel tiles), LDS key-set
     entries per workgroup, and the probe scheme (reference multiply-xor hash
     over a power-of-two table, or a 64-bit mix with multiply-shift range);
   * update: workgroup size (waves per workgroup, one block per wave) and

@@ -1,5 +1,5 @@
-// Allocation for gfx950: band walk -> wave64 dedup -> workgroup LDS pre-hash ->
-// table insert (phase A), then a deterministic sorted index assignment
+// Allocation for gfx950: band walk -> wave64 dedup -> workgroup LDS key set
+// (one per work unit) -> table insert (phase A), then a deterministic sorted index assignment
 // (phase B). Also the reference-semantics scatter update, which shares the
 // band walk.
 //
@@ -33,7 +33,7 @@ constexpr int kKeysPerLane = 4;     // distinct keys a lane buffers before a flu
 constexpr int kAllocWavesPerSimd = 6;
 constexpr int kWgList = 512;        // LDS touched-list capacity per workgroup
 constexpr int kWgNew = 256;         // LDS new-key-list capacity per workgroup
-constexpr int kMaxSet = 1024;       // LDS pre-hash capacity
+constexpr int kMaxSet = 1024;       // LDS key-set capacity
 
 enum SrcKind : int { kSrcPoints = 0, kSrcDepth = 1, kSrcCoords = 2 };
 
@@ -64,7 +64,7 @@ namespace {
 
 // Table and list accesses go through the global address space (global, not
 // flat, instructions: a flat access also counts against the LDS wait counter,
-// so every LDS wait of the pre-hash would wait for outstanding table probes).
+// so every LDS wait of the key set would wait for outstanding table probes).
 #define OSN_GLOBAL __attribute__((address_space(1)))
 template <typename T>
 __device__ inline T OSN_GLOBAL* glob(T* p) {
@@ -186,26 +186,22 @@ struct TileCtx {
     int32_t stamp;
 };
 
-// One batch of up to 64 distinct keys, one per lane (has = lane holds one).
+// Slot of a key in the workgroup set: the low bits of the three block
+// coordinates interleaved (z in bits 0-2, y in 3-5, x from 6 up), so the
+// blocks of a unit's few-block neighbourhood land in distinct slots; shifts
+// and exclusive-ors only.
+__device__ inline uint32_t set_slot(int64_t key, int set_size) {
+    const uint32_t lo = uint32_t(uint64_t(key)), hi = uint32_t(uint64_t(key) >> 32);
+    return (lo ^ (lo >> 18) ^ (hi >> 4)) & uint32_t(set_size - 1);
+}
+
+// One batch of up to 64 distinct keys, one per lane (go = lane holds one),
+// against the global table: find or insert, the touched and new lists, and
+// the per-workgroup statistics.
 template <int MODE, bool TOUCH>
-__device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
+__device__ void table_keys(const TileCtx& c, bool go, int64_t key) {
     const VolDev& v = *c.v;
     WgShared& s = *c.s;
-    // Workgroup pre-hash: only the lane that claims a key in LDS goes on to the
-    // global table; copies from the other waves of the workgroup stop here.
-    bool go = has;
-    if (has && c.set_size > 0) {
-        uint32_t h = uint32_t((uint64_t(key) * 0x9E3779B97F4A7C15ull) >> 40) & (c.set_size - 1);
-        for (int p = 0; p < 8; ++p) {
-            const unsigned long long prev =
-                atomicCAS(reinterpret_cast<unsigned long long*>(&s.set[h]),
-                          (unsigned long long)kEmptyKey, (unsigned long long)key);
-            if (prev == (unsigned long long)kEmptyKey) break;  // ours
-            if (prev == (unsigned long long)key) { go = false; break; }
-            h = (h + 1) & (c.set_size - 1);
-        }
-    }
-
     bool inserted = false, found = false;
     uint32_t slot = 0;
     uint32_t steps = 0;
@@ -243,8 +239,10 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
     }
 
     // per-workgroup statistics (rare or measurement-only), one wave-level add
-    const unsigned long long wd = wave_sum_u64(drops);
-    if (__lane_id() == 0 && wd) atomicAdd(&s.drops, wd);
+    if (__ballot(drops != 0)) {
+        const unsigned long long wd = wave_sum_u64(drops);
+        if (__lane_id() == 0) atomicAdd(&s.drops, wd);
+    }
     if (MODE == kAllocCountCas) {
         const unsigned long long wc = wave_sum_u64(cas);
         if (__lane_id() == 0 && wc) atomicAdd(&s.cas, wc);
@@ -280,6 +278,57 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
     if (MODE != kAllocNoCas)
         wave_append<kWgNew>(inserted, TouchEntry{key, slot, kIdxPending}, &s.n_new, s.new_keys,
                             &c.lc->new_count, v.new_keys);
+}
+
+// One batch of up to 64 distinct keys of a wave (has = lane holds one). With
+// a workgroup set, a key is only placed in it here (one LDS compare-and-swap,
+// a key the set already holds stops there); the set's keys reach the table
+// together in flush_set at the end of the work unit, so each distinct key of
+// the unit costs one table lookup and the waves' batches of a few keys cost
+// no table round trip. A key the crowded set cannot take after 8 probes goes
+// to the table at once. Without a set every batch goes to the table.
+template <int MODE, bool TOUCH>
+__device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
+    if (c.set_size == 0) {
+        table_keys<MODE, TOUCH>(c, has, key);
+        return;
+    }
+    WgShared& s = *c.s;
+    bool spill = false;
+    if (has) {
+        uint32_t h = set_slot(key, c.set_size);
+        spill = true;
+        for (int p = 0; p < 8; ++p) {
+            const unsigned long long prev =
+                atomicCAS(reinterpret_cast<unsigned long long*>(&s.set[h]),
+                          (unsigned long long)kEmptyKey, (unsigned long long)key);
+            if (prev == (unsigned long long)kEmptyKey || prev == (unsigned long long)key) {
+                spill = false;
+                break;
+            }
+            h = (h + 1) & (c.set_size - 1);
+        }
+    }
+    if (__ballot(spill)) table_keys<MODE, TOUCH>(c, spill, key);
+}
+
+// End of a work unit: every key the set holds goes to the table, the set's
+// entries taken one per thread (whole waves, so the list appends' ballots see
+// every lane), and the set is left empty for the next unit. Every thread of
+// the workgroup calls it.
+template <int MODE, bool TOUCH>
+__device__ void flush_set(const TileCtx& c) {
+    if (c.set_size == 0) return;
+    WgShared& s = *c.s;
+    __syncthreads();
+    for (int base = 0; base < c.set_size; base += blockDim.x) {
+        const int i = base + (int)threadIdx.x;
+        const int64_t key = i < c.set_size ? s.set[i] : kEmptyKey;
+        const bool has = key != kEmptyKey;
+        if (__ballot(has)) table_keys<MODE, TOUCH>(c, has, key);
+        if (i < c.set_size) s.set[i] = kEmptyKey;
+    }
+    __syncthreads();
 }
 
 // Drain every lane's buffered keys: repeatedly take the first live key of the
@@ -599,7 +648,7 @@ __device__ inline bool backproject_pixel(const PointSrc& src, int u, int w, floa
 
 // Depth images are walked in work units of sub x sub tiles of 16x16 pixels
 // (sub = 1 for a single frame, whose 16x16 tiles already fill the chip; 2 in
-// batches): the workgroup keeps its pre-hash across the unit's tiles, each of
+// batches): the workgroup collects the unit's keys in its set, each of
 // its four waves takes an 8x8 quarter of a tile, and the depth of the next
 // tile is loaded before the current one is walked. With src.tiled set, each
 // pixel is also stored to the update's 8x4-tiled copy: a wave's 8x8 quarter is
@@ -637,7 +686,7 @@ __device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t*
     const bool has = i < src.n;
     if (SRC == kSrcCoords) {
         // Given keys have no spatial coherence to exploit within a wave, so
-        // each lane goes straight to the workgroup pre-hash with its key.
+        // each lane goes straight to the workgroup key set with its key.
         const int64_t key =
             has ? pack_coord(glob(coords)[i * 3 + 0], glob(coords)[i * 3 + 1], glob(coords)[i * 3 + 2])
                 : kEmptyKey;
@@ -660,12 +709,6 @@ __device__ void wg_init(WgShared& s, int set_size) {
         s.n_new = s.n_touched = 0;
         s.drops = s.cas = s.lookups = s.lane_steps = s.wave_steps = 0;
     }
-}
-
-__device__ void clear_set(WgShared& s, int set_size) {
-    __syncthreads();
-    for (int i = threadIdx.x; i < set_size; i += blockDim.x) s.set[i] = kEmptyKey;
-    __syncthreads();
 }
 
 template <int MODE>
@@ -710,9 +753,9 @@ __global__ __launch_bounds__(kAllocThreads, kAllocWavesPerSimd) void k_alloc(Vol
     __syncthreads();
     const TileCtx c{&v, &v.ctr->lists[call.parity], &s, set_size, call.stamp};
     for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        if (tile != (int)blockIdx.x) clear_set(s, set_size);
         if (SRC == kSrcDepth) alloc_depth_unit<MODE, TOUCH>(c, src, tile, 1);
         else alloc_tile<MODE, TOUCH, SRC>(c, src, coords, tile);
+        flush_set<MODE, TOUCH>(c);
     }
     flush_lists(v, c.lc, s);
     flush_stats<MODE>(v, s);
@@ -753,9 +796,9 @@ __global__ __launch_bounds__(kAllocThreads, kAllocWavesPerSimd) void k_alloc_bat
             src = load_const(&st->src);
             call = batch_call(b, sc, k);
         }
-        clear_set(s, set_size);
         const TileCtx c{&v, &v.ctr->lists[call.parity], &s, set_size, call.stamp};
-        alloc_depth_unit<MODE, true>(c, src, local, kBatchDepthSub);
+        alloc_depth_unit<MODE, true>(c, src, local, b.depth_sub);
+        flush_set<MODE, true>(c);
     }
     if (cur >= 0) {
         flush_lists(v, &v.ctr->lists[call.parity], s);

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -884,6 +885,7 @@ struct OsnTsdfHipBatch {
     std::vector<int64_t> base_calls;
     int32_t alloc_grid = 0, update_grid = 0, update_threads = 256, lds_set = 1024;
     int32_t alloc_mode = kAllocNormal;  // instrumentation of the allocate kernel
+    int32_t depth_sub = kBatchDepthSub;  // allocate work-unit size, set with the schedule
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // mesh extraction every `extract_every` steps (0: never), per scene into
@@ -965,6 +967,7 @@ struct OsnTsdfHipBatch {
         b.upd = d_upd;
         b.step = step;
         b.n_scenes = (int32_t)vols.size();
+        b.depth_sub = depth_sub;
         return b;
     }
     hipError_t enqueue_step(const BatchDev& b) {
@@ -1057,6 +1060,19 @@ int32_t osn_tsdf_hip_batch_set_schedule(OsnTsdfHipBatch* b, int32_t n_steps,
     b->drop_graph();
     std::vector<SceneStep> hs((size_t)n_steps * n);
     std::vector<int32_t> tiles(n, 0);
+    // the work-unit size: 2x2 tiles unless that leaves the launch short of units
+    std::vector<int32_t> widest(n, 0);
+    for (int k = 0; k < n_steps; ++k)
+        for (int s = 0; s < n; ++s) {
+            const size_t i = (size_t)k * n + s;
+            if (!d_depths[i]) continue;
+            if (!valid_frame(d_depths[i], &frames[i])) return 1;
+            widest[s] = std::max(widest[s], depth_units(frames[i].width, frames[i].height,
+                                                        kBatchDepthSub));
+        }
+    int64_t units = 0;
+    for (int s = 0; s < n; ++s) units += widest[s];
+    const int32_t sub = units < kBatchMinUnits ? 1 : kBatchDepthSub;
     for (int k = 0; k < n_steps; ++k)
         for (int s = 0; s < n; ++s) {
             const size_t i = (size_t)k * n + s;
@@ -1067,7 +1083,7 @@ int32_t osn_tsdf_hip_batch_set_schedule(OsnTsdfHipBatch* b, int32_t n_steps,
             st.active = 1;
             st.src = depth_src(d_depths[i], &frames[i]);
             st.cam = proj_cam(nullptr, &frames[i]);  // tiled copy, set below
-            st.tiles = depth_units(frames[i].width, frames[i].height, kBatchDepthSub);
+            st.tiles = depth_units(frames[i].width, frames[i].height, sub);
             if (st.tiles > tiles[s]) tiles[s] = st.tiles;
         }
     // Each volume's tiled depth copy is sized for its largest frame before any
@@ -1097,6 +1113,7 @@ int32_t osn_tsdf_hip_batch_set_schedule(OsnTsdfHipBatch* b, int32_t n_steps,
     b->n_steps = n_steps;
     b->cursor = 0;
     b->total_tiles = prefix[n];
+    b->depth_sub = sub;
     return 0;
 }
 

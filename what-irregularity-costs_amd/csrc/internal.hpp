@@ -39,7 +39,7 @@ enum AllocMode : int32_t {
 struct AllocOpts {
     int32_t mode = kAllocNormal;
     bool track_touched = false;
-    int32_t lds_set = 1024;   // per-workgroup pre-hash entries (0 disables)
+    int32_t lds_set = 1024;   // per-workgroup LDS key-set entries (0: every batch to the table)
     int32_t grid = 0;         // workgroups; 0 = one per tile up to a cap
 };
 
@@ -49,12 +49,16 @@ struct SceneStep {
     PointSrc src;
     ProjCam cam;
     int32_t active;
-    int32_t tiles;      // depth work units of this scene's image (kBatchDepthSub)
+    int32_t tiles;      // depth work units of this scene's image (the batch's depth_sub)
 };
 
 // Depth images are allocated in work units of sub x sub tiles of 16x16
-// pixels, one workgroup each; batches use kBatchDepthSub.
+// pixels, one workgroup each. Batches use kBatchDepthSub, unless the whole
+// batch would then have fewer than kBatchMinUnits units (a single scene or a
+// few small ones): a unit per 16x16 tile then, so the launch still fills the
+// chip.
 constexpr int kBatchDepthSub = 2;
+constexpr int kBatchMinUnits = 2048;
 __host__ __device__ inline int32_t depth_units(int32_t W, int32_t H, int32_t sub) {
     const int32_t span = 16 * sub;
     return ((W + span - 1) / span) * ((H + span - 1) / span);
@@ -77,6 +81,7 @@ struct BatchDev {
     UpdParams* upd;              // per scene: the update's parameters, written each step
     int32_t step;
     int32_t n_scenes;
+    int32_t depth_sub;           // work units of depth_sub x depth_sub 16x16 tiles
 };
 
 // Call number c of a volume (0-based) stamps its blocks with c + 1 and
