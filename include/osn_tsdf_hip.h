@@ -218,6 +218,24 @@ int32_t osn_tsdf_hip_read_voxels_f64(const OsnTsdfHipVolume* v, int32_t first, i
 int32_t osn_tsdf_hip_extract_mesh_device(OsnTsdfHipVolume* v, float* d_posnor, uint8_t* d_rgb,
                                          int32_t vert_cap, float min_weight, float iso,
                                          int32_t* d_count);
+/* osn_tsdf_hip_extract_mesh_device without any device allocation. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The per-block scratch (neighbour indices, vertex
+ * counts, offsets, chunk sums, the scan's arrival counter) lives in one
+ * library-wide device array of OSN_TSDF_HIP_NOALLOC_SCRATCH_BYTES on each
+ * device, instead of the volume's own, grown on demand; neighbour indices are
+ * looked up afresh on every call. Same arguments and results as the
+ * allocating call, and byte-identical output; returns 4 (nothing launched)
+ * when the volume's pool exceeds OSN_TSDF_HIP_NOALLOC_MAX_BLOCKS blocks.
+ * Every call on one device shares the array: calls that overlap on that
+ * device (different streams, no synchronisation between them) race. */
+#define OSN_TSDF_HIP_NOALLOC_MAX_BLOCKS (1 << 21)
+#define OSN_TSDF_HIP_NOALLOC_SCRATCH_BYTES 83894784 /* 40 B per block + chunk sums + counter */
+int32_t osn_tsdf_hip_extract_mesh_device_noalloc(OsnTsdfHipVolume* v, float* d_posnor,
+                                                 uint8_t* d_rgb, int32_t vert_cap,
+                                                 float min_weight, float iso,
+                                                 int32_t* d_count);
 /* Cumulative probe statistics (see allocate_probe_depth) and the number of
  * distinct keys sent to the table; any pointer may be NULL. Synchronises. */
 int32_t osn_tsdf_hip_probe_stats(const OsnTsdfHipVolume* v, uint64_t* hist64,

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -73,6 +74,7 @@ struct OsnTsdfHipVolume {
     int64_t calls = 0;  // allocate calls since create/reset: call c stamps c + 1, uses list set c & 1
     bool in_batch = false;
     hipStream_t batch_stream = nullptr;  // the batch's stream while in a batch
+    int device = 0;                      // the device current at create
     // extraction scratch, grown on demand
     int32_t* ex_vcount = nullptr;
     int32_t* ex_voff = nullptr;
@@ -120,6 +122,7 @@ bool OsnTsdfHipVolume::fail(const char* what, hipError_t e) {
 
 bool OsnTsdfHipVolume::alloc() {
     const uint32_t cap = d.capacity;
+    HIP_OK(hipGetDevice(&device));
     HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     table_alloc = cap + (reference_shape ? kScratchSlots : 0u);
     HIP_OK(hipMalloc(&d.table, (size_t)table_alloc * sizeof(HashEntry)));
@@ -645,6 +648,7 @@ int32_t osn_tsdf_hip_extract_mesh_device(OsnTsdfHipVolume* v, float* d_posnor, u
     a.block_vcount = v->ex_vcount;
     a.block_voff = v->ex_voff;
     a.block_nb = v->ex_nb;
+    a.nb_cache = 1;
     a.chunk_sums = v->ex_chunks;
     a.scan_flag = v->ex_flag_count;
     a.posnor = d_posnor;
@@ -656,6 +660,96 @@ int32_t osn_tsdf_hip_extract_mesh_device(OsnTsdfHipVolume* v, float* d_posnor, u
     hipError_t err = launch_extract(v->d, a, false, v->stream);
     if (err == hipSuccess) err = launch_extract(v->d, a, true, v->stream);
     v->stage_end(3, e);
+    if (err != hipSuccess) { v->fail("extract launch", err); return 2; }
+    return 0;
+}
+
+}  // extern "C"
+
+// ---- extraction that allocates nothing ----------------------------------
+//
+// The per-block scratch of a device-resident extraction (neighbour indices,
+// vertex counts, offsets, chunk sums, the scan's arrival counter) for volumes
+// of up to OSN_TSDF_HIP_NOALLOC_MAX_BLOCKS pool blocks, in one file-scope
+// device array. Each device that loads the library holds its own copy until
+// the process ends; every no-allocation call on a device shares it.
+namespace {
+
+constexpr size_t kNaBlocks = OSN_TSDF_HIP_NOALLOC_MAX_BLOCKS;
+constexpr size_t kNaNbOff = 0;                                  // 8 x int32 per block
+constexpr size_t kNaCountOff = kNaNbOff + kNaBlocks * 8 * 4;    // int32 per block
+constexpr size_t kNaVoffOff = kNaCountOff + kNaBlocks * 4;      // int32 per block
+constexpr size_t kNaChunkOff = kNaVoffOff + kNaBlocks * 4;      // int32 per 1024 blocks, + 2
+constexpr size_t kNaFlagOff =
+    (kNaChunkOff + (kNaBlocks / kScanChunk + 2) * 4 + 255) / 256 * 256;  // 2 x int32
+constexpr size_t kNaBytes = kNaFlagOff + 256;
+static_assert(kNaBytes == OSN_TSDF_HIP_NOALLOC_SCRATCH_BYTES, "header states the scratch size");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// The array's address on each device, looked up once with that device
+// current: in host code `scratch` only names the symbol, it is not a device
+// address.
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+unsigned char* g_na_ptr[kNaMaxDevices] = {};
+
+unsigned char* noalloc_scratch(int device) {
+    if (device < 0 || device >= kNaMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (g_na_ptr[device]) return g_na_ptr[device];
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+    if (cur != device && hipSetDevice(device) != hipSuccess) return nullptr;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) hipSetDevice(cur);
+    if (e != hipSuccess || !ptr) {
+        std::fprintf(stderr, "[osn_tsdf_hip] scratch symbol lookup failed: %s\n",
+                     hipGetErrorString(e));
+        return nullptr;
+    }
+    g_na_ptr[device] = static_cast<unsigned char*>(ptr);
+    return g_na_ptr[device];
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t osn_tsdf_hip_extract_mesh_device_noalloc(OsnTsdfHipVolume* v, float* d_posnor,
+                                                 uint8_t* d_rgb, int32_t vert_cap,
+                                                 float min_weight, float iso,
+                                                 int32_t* d_count) {
+    if (!v || !d_posnor || vert_cap <= 0 || !d_count) return 1;
+    if (!v->ok) return 2;
+    if ((size_t)v->d.pool_capacity > kNaBlocks) {
+        std::fprintf(stderr,
+                     "[osn_tsdf_hip] extract_mesh_device_noalloc: pool of %d blocks exceeds the "
+                     "%zu-block scratch\n", v->d.pool_capacity, kNaBlocks);
+        return 4;
+    }
+    unsigned char* base = noalloc_scratch(v->device);
+    if (!base) return 2;
+    // a batch writes this volume on its own stream: let that work finish first
+    if (v->batch_stream && hipStreamSynchronize(v->batch_stream) != hipSuccess) return 2;
+    ExtractArgs a{};
+    a.min_weight = min_weight;
+    a.iso = iso;
+    a.n_blocks = -1;  // read from the device: no host synchronisation
+    a.block_nb = reinterpret_cast<int32_t*>(base + kNaNbOff);
+    a.block_vcount = reinterpret_cast<int32_t*>(base + kNaCountOff);
+    a.block_voff = reinterpret_cast<int32_t*>(base + kNaVoffOff);
+    a.chunk_sums = reinterpret_cast<int32_t*>(base + kNaChunkOff);
+    a.scan_flag = reinterpret_cast<int32_t*>(base + kNaFlagOff);
+    a.nb_cache = 0;    // the scratch holds another call's indices: look every neighbour up
+    a.clear_flag = 1;  // and start the arrival counter at zero
+    a.posnor = d_posnor;
+    a.rgb = d_rgb;
+    a.cap = vert_cap;
+    a.count_out = d_count;
+    hipError_t err = launch_extract(v->d, a, false, v->stream);
+    if (err == hipSuccess) err = launch_extract(v->d, a, true, v->stream);
     if (err != hipSuccess) { v->fail("extract launch", err); return 2; }
     return 0;
 }
@@ -674,6 +768,7 @@ int32_t osn_tsdf_hip_extract_mesh(OsnTsdfHipVolume* v, float* buffer_posnor, uin
     a.block_vcount = v->ex_vcount;
     a.block_voff = v->ex_voff;
     a.block_nb = v->ex_nb;
+    a.nb_cache = 1;
     a.chunk_sums = v->ex_chunks;
     a.scan_flag = v->ex_flag_count;
     a.cap = vert_cap;
@@ -1210,6 +1305,7 @@ int32_t osn_tsdf_hip_batch_set_extract(OsnTsdfHipBatch* b, float* const* d_posno
         a.block_vcount = v->ex_vcount;
         a.block_voff = v->ex_voff;
         a.block_nb = v->ex_nb;
+        a.nb_cache = 1;
         a.chunk_sums = v->ex_chunks;
         a.scan_flag = v->ex_flag_count;
         a.posnor = d_posnor[s];

@@ -43,7 +43,6 @@ constexpr int kWaveThreads = 64;     // count and emit: one wave per block, many
 template <int STORE>
 constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : (STORE == kStoreF16 ? 6 : 4); }
 constexpr int kCube = 9 * 9 * 9;
-constexpr int kScanChunk = 1024;
 constexpr int kScanGrid = 256;      // scan workgroups (grid-stride over chunks)
 constexpr int kSerialChunks = 64;   // up to this many chunks one workgroup scans alone
 constexpr uint32_t kScanGridBatch = 32;  // the same per volume of a batch
@@ -532,7 +531,8 @@ __device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t firs
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
-    const int32_t cached = global(v.ctr)->nb_blocks;
+    const int32_t cached = a.nb_cache ? global(v.ctr)->nb_blocks : 0;
+    if (a.clear_flag && first == 0 && lane == 0) *global(a.scan_flag) = 0;
     if constexpr (STORE != kStoreSums) {
         // blocks b and b + stride together
         __shared__ Cube<STORE> c2[2];
@@ -587,7 +587,7 @@ __device__ void scan_chunks(const VolDev& v, const ExtractArgs& a) {
     const int32_t n = n_blocks_of(v, a);
     const int n_chunks = (n + kScanChunk - 1) / kScanChunk;
     // the count pass that just ran holds every block's neighbour indices
-    if (blockIdx.x == 0 && threadIdx.x == 0) global(v.ctr)->nb_blocks = n;
+    if (a.nb_cache && blockIdx.x == 0 && threadIdx.x == 0) global(v.ctr)->nb_blocks = n;
     if (n == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             *a.count_out = 0;

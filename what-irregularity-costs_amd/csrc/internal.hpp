@@ -91,6 +91,10 @@ __device__ inline CallInfo batch_call(const BatchDev& b, int sc, int k) {
     return CallInfo{c + 1, c & 1};
 }
 
+// Extraction scans the per-block vertex counts in chunks of this many blocks
+// (chunk_sums holds one total per chunk).
+constexpr int kScanChunk = 1024;
+
 struct ExtractArgs {
     float min_weight;
     float iso;
@@ -104,6 +108,10 @@ struct ExtractArgs {
     uint8_t* rgb;          // device output, 3 bytes per vertex, may be null
     int32_t cap;           // output capacity in vertices
     int32_t* count_out;    // device int32: vertex count or -1 on overflow
+    int32_t nb_cache;      // 1: block_nb is the volume's own neighbour cache (reuse the
+                           // indices of blocks below Counters.nb_blocks, then advance it);
+                           // 0: look every neighbour up and leave nb_blocks alone
+    int32_t clear_flag;    // 1: the count pass zeroes scan_flag before the scan reads it
 };
 
 // Index assignment works in tiles of kSortTile new keys: level 0 sorts each

@@ -112,6 +112,7 @@ def lib():
         "osn_tsdf_hip_read_voxels": (_I, [_VP, _I, _I, _VP, _VP]),
         "osn_tsdf_hip_read_voxels_f64": (_I, [_VP, _I, _I, _VP, _VP]),
         "osn_tsdf_hip_extract_mesh_device": (_I, [_VP, _VP, _VP, _I, _F, _F, _VP]),
+        "osn_tsdf_hip_extract_mesh_device_noalloc": (_I, [_VP, _VP, _VP, _I, _F, _F, _VP]),
         "osn_tsdf_hip_probe_stats": (_I, [_VP, _VP, _VP, _VP, _VP]),
         "osn_tsdf_hip_set_timing": (_I, [_VP, _I]),
         "osn_tsdf_hip_read_timing": (_I, [_VP, _VP, _VP]),
@@ -233,6 +234,11 @@ class HipVolume:
     def extract_device(self, d_posnor, d_rgb, cap, min_weight, iso, d_count) -> int:
         return self._L.osn_tsdf_hip_extract_mesh_device(self.h, _dptr(d_posnor), _dptr(d_rgb),
                                                         cap, min_weight, iso, _dptr(d_count))
+
+    def extract_device_noalloc(self, d_posnor, d_rgb, cap, min_weight, iso, d_count) -> int:
+        """extract_device with the library's static scratch: no device allocation."""
+        return self._L.osn_tsdf_hip_extract_mesh_device_noalloc(
+            self.h, _dptr(d_posnor), _dptr(d_rgb), cap, min_weight, iso, _dptr(d_count))
 
     def reset(self) -> int:
         return self._L.osn_tsdf_hip_reset(self.h)
