@@ -164,3 +164,45 @@ def test_autotuner_scores_update_shapes_over_two_rounds():
     n_update_trials = len(autotune.UPDATE_THREADS) * len(autotune.UPDATE_GRIDS)
     assert len(log["trials"]) == n_alloc_trials + autotune.UPDATE_ROUNDS * n_update_trials
     assert FakeBatch.calls[(256, 4096)] == autotune.UPDATE_ROUNDS == 2
+
+
+@pytest.mark.parametrize("scale,voxel,trunc", [
+    (5.0, 0.01, 0.04), (100.0, 0.01, 0.04), (8.0, 0.02, 0.08), (12.0, 0.005, 0.02),
+    (3.0, 0.01, 0.025), (30.0, 0.05, 0.11),
+])
+def test_last_sample_cull_pretest_agrees_with_reference(scale, voxel, trunc):
+    """The device decides sample S's cull from the squared distance e2 of its
+    voxel centre against T^2 (1 -+ 2^-19), T = dist + trunc, and evaluates the
+    reference expression dist - sqrt(e2) < -trunc (cpp/src/volume.cpp:183-189)
+    only between the two bounds. In exact FP32 emulation, neither bound ever
+    disagrees with the reference expression."""
+    rng = np.random.default_rng(int(scale * 7) + int(voxel * 1e4))
+    n = 300000
+    voxel, trunc = f32(voxel), f32(trunc)
+    inv = f32(1.0) / voxel
+    cam = rng.uniform(-scale, scale, 3).astype(f32)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    r = np.concatenate([rng.uniform(0.0005, 2 * float(trunc), n // 4),
+                        rng.uniform(0.1, scale, n - n // 4)])
+    p = (cam + dirs * r[:, None]).astype(f32)
+    d = (p - cam).astype(f32)
+    dist = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]).astype(f32))
+    keep_pt = dist > f32(1e-6)
+    p, d, dist = p[keep_pt], d[keep_pt], dist[keep_pt]
+    u = (d / dist[:, None]).astype(f32)
+    S = max(1, math.ceil(trunc * inv))
+    t = f32(S) * voxel
+    v = np.floor(((p + u * t).astype(f32) * inv).astype(f32))
+    c = ((v + f32(0.5)) * voxel).astype(f32)
+    e = (c - cam).astype(f32)
+    e2 = ((e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]).astype(f32) + e[:, 2] * e[:, 2]).astype(f32)
+    reference = (dist - np.sqrt(e2).astype(f32)).astype(f32) < -trunc
+    T = (dist + trunc).astype(f32)
+    TT = (T * T).astype(f32)
+    keep = e2 < (TT * f32(1.0 - 2.0 ** -19)).astype(f32)
+    cull = e2 > (TT * f32(1.0 + 2.0 ** -19)).astype(f32)
+    assert not np.any(keep & reference)
+    assert not np.any(cull & ~reference)
+    assert (~keep & ~cull).mean() < 0.01  # the exact expression is rarely needed
+    assert reference.any() and (~reference).any()
