@@ -106,6 +106,8 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long x) {
 
 struct WgShared {
     int64_t set[kMaxSet];
+    int64_t dense[kAllocThreads];  // a unit's keys, compacted for the table step
+    int32_t n_dense;
     TouchEntry new_keys[kWgNew];
     TouchEntry touched[kWgList];
     int32_t n_new, n_touched;
@@ -312,10 +314,12 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
     if (__ballot(spill)) table_keys<MODE, TOUCH>(c, spill, key);
 }
 
-// End of a work unit: every key the set holds goes to the table, the set's
-// entries taken one per thread (whole waves, so the list appends' ballots see
-// every lane), and the set is left empty for the next unit. Every thread of
-// the workgroup calls it.
+// End of a work unit: every key the set holds goes to the table, and the set
+// is left empty for the next unit. The set's keys are first compacted into a
+// dense list (one LDS atomic per wave), so a unit's few dozen keys fill one
+// wave's lanes at the table instead of a few lanes of every wave; a key past
+// the list's capacity goes to the table at once. Every thread of the
+// workgroup calls it.
 template <int MODE, bool TOUCH>
 __device__ void flush_set(const TileCtx& c) {
     if (c.set_size == 0) return;
@@ -325,10 +329,27 @@ __device__ void flush_set(const TileCtx& c) {
         const int i = base + (int)threadIdx.x;
         const int64_t key = i < c.set_size ? s.set[i] : kEmptyKey;
         const bool has = key != kEmptyKey;
-        if (__ballot(has)) table_keys<MODE, TOUCH>(c, has, key);
+        const unsigned long long m = __ballot(has);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            int at = 0;
+            if (__lane_id() == leader) at = atomicAdd(&s.n_dense, __popcll(m));
+            at = __builtin_amdgcn_readlane(at, leader) + __popcll(m & lanemask_lt());
+            const bool spill = has && at >= kAllocThreads;
+            if (has && !spill) s.dense[at] = key;
+            if (__ballot(spill)) table_keys<MODE, TOUCH>(c, spill, key);
+        }
         if (i < c.set_size) s.set[i] = kEmptyKey;
     }
     __syncthreads();
+    const int n = min(s.n_dense, kAllocThreads);
+    for (int b0 = (int)(threadIdx.x & ~63); b0 < n; b0 += blockDim.x) {
+        const int i = b0 + __lane_id();
+        const bool has = i < n;
+        table_keys<MODE, TOUCH>(c, has, has ? s.dense[i] : kEmptyKey);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s.n_dense = 0;
 }
 
 // Drain every lane's buffered keys: repeatedly take the first live key of the
@@ -734,6 +755,7 @@ __device__ void wg_init(WgShared& s, int set_size) {
     for (int i = threadIdx.x; i < kProbeHistBins; i += blockDim.x) s.hist[i] = 0;
     if (threadIdx.x == 0) {
         s.n_new = s.n_touched = 0;
+        s.n_dense = 0;
         s.drops = s.cas = s.lookups = s.lane_steps = s.wave_steps = 0;
     }
 }
