@@ -41,7 +41,7 @@ def test_harness_cells_csv_and_export(tmp_path):
     rows = list(csv.DictReader(open(out_csv)))
     assert list(rows[0].keys())[:6] == ["run", "device", "device_name", "sm", "sm_count", "cell"]
     assert {(x["cell"], x["arm"], x["stage"]) for x in rows} == {
-        (c, a, s) for c in ("base", "plane-320k") for a in ("hip", "hip-lds")
+        (c, a, s) for c in ("base", "plane-320k") for a in ("hip", "hip-noset")
         for s in ("allocate", "update")} | {(c, "hip", "extract-hip-only")
                                             for c in ("base", "plane-320k")}
     blocks = {x["cell"]: int(x["blocks"]) for x in rows}
@@ -51,7 +51,7 @@ def test_harness_cells_csv_and_export(tmp_path):
         assert int(x["n"]) == 4
     for cell, verts in (("base", 846288), ("plane-320k", 950424)):
         a = F.read_osn1(str(ex / f"{cell}.hip.raw"))
-        b = F.read_osn1(str(ex / f"{cell}.hip-lds.raw"))
+        b = F.read_osn1(str(ex / f"{cell}.hip-noset.raw"))
         assert len(a) == len(b) == verts
         assert np.abs(a[:, :3] - b[:, :3]).max() < 1e-5
 

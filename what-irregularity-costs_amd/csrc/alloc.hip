@@ -878,16 +878,33 @@ constexpr uint32_t kSpinLimit = 1u << 22;   // about 0.5 s of sleeps
 struct AssignLds {
     int64_t key[kSortTile];
     uint32_t slot[kSortTile];
+    int64_t key2[kSortTile];   // the sort's merge levels alternate between the two
+    uint32_t slot2[kSortTile];
     int32_t ticket;
     int32_t flag;
     int32_t corank[2];
     unsigned long long drops;
 };
 
-// Sort key/slot[0, cnt) in place: rank sort for short lists (one barrier),
-// bitonic over the padded power of two otherwise. Real keys are distinct; the
-// padding compares above any real entry through the slot tie-break, since a
-// real key may itself equal INT64_MAX.
+__device__ inline bool entry_less(int64_t ka, uint32_t sa, int64_t kb, uint32_t sb) {
+    return ka < kb || (ka == kb && sa < sb);
+}
+
+__device__ inline int64_t shfl_xor64(int64_t x, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)(uint64_t)x, m);
+    const int hi = __shfl_xor((int)(uint32_t)((uint64_t)x >> 32), m);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+// Sort key/slot[0, cnt) in place by (key, slot). Real keys are distinct; the
+// padding (INT64_MAX, all-ones slot) compares above any real entry through
+// the slot, since a real key may itself equal INT64_MAX. Up to 256 entries: a
+// rank sort (one barrier). Above: the list padded to a power of two P is cut
+// into runs of 64 that waves sort in registers with a bitonic network of
+// lane exchanges (no barriers), then runs are merged pairwise in LDS, level
+// by level (one barrier each): every thread finds where its stretch of the
+// merged output starts by a binary search over the two runs (merge path)
+// and merges that stretch sequentially, ties taken from the first run.
 __device__ void lds_sort(AssignLds& s, int cnt) {
     if (cnt <= 256) {
         const bool has = (int)threadIdx.x < cnt;
@@ -911,21 +928,67 @@ __device__ void lds_sort(AssignLds& s, int cnt) {
         s.slot[i] = 0xffffffffu;
     }
     __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = threadIdx.x; t < (P >> 1); t += blockDim.x) {
-                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));  // lower index of the pair
-                const int ixj = i + j;
-                const int64_t x = s.key[i], y = s.key[ixj];
-                const uint32_t sx = s.slot[i], sy = s.slot[ixj];
-                const bool gt = x > y || (x == y && sx > sy);
-                if (gt == ((i & k) == 0)) {
-                    s.key[i] = y; s.key[ixj] = x;
-                    s.slot[i] = sy; s.slot[ixj] = sx;
-                }
+    const int lane = __lane_id();
+    const int waves = blockDim.x >> 6;
+    for (int r = threadIdx.x >> 6; r < (P >> 6); r += waves) {
+        int64_t k = s.key[r * 64 + lane];
+        uint32_t sl = s.slot[r * 64 + lane];
+        for (int kk = 2; kk <= 64; kk <<= 1) {
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                const int64_t ok = shfl_xor64(k, j);
+                const uint32_t osl = (uint32_t)__shfl_xor((int)sl, j);
+                // the pair's lower position keeps the smaller entry in an
+                // ascending block (lane & kk clear) and the larger otherwise
+                const bool keep_min = ((lane & j) != 0) == ((lane & kk) != 0);
+                const bool other_less = entry_less(ok, osl, k, sl);
+                if (keep_min == other_less) { k = ok; sl = osl; }
             }
-            __syncthreads();
         }
+        s.key[r * 64 + lane] = k;
+        s.slot[r * 64 + lane] = sl;
+    }
+    __syncthreads();
+    int64_t* ka = s.key;
+    uint32_t* sa = s.slot;
+    int64_t* kb = s.key2;
+    uint32_t* sb = s.slot2;
+    const int per = P > (int)blockDim.x ? P / (int)blockDim.x : 1;  // outputs per thread
+    for (int run = 64; run < P; run <<= 1) {
+        for (int t = threadIdx.x; t * per < P; t += blockDim.x) {
+            const int og = t * per;
+            const int a0 = og / (2 * run) * (2 * run), b0 = a0 + run;
+            const int o = og - a0;
+            int lo = max(0, o - run), hi = min(o, run);
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                // A[mid] goes out before B[o - mid - 1]: it is among the first o
+                if (!entry_less(ka[b0 + o - mid - 1], sa[b0 + o - mid - 1], ka[a0 + mid], sa[a0 + mid]))
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            int ia = lo, ib = o - lo;
+            for (int q = 0; q < per; ++q) {
+                const bool take_a = ib >= run ||
+                                    (ia < run && !entry_less(ka[b0 + ib], sa[b0 + ib], ka[a0 + ia],
+                                                             sa[a0 + ia]));
+                const int from = take_a ? a0 + ia : b0 + ib;
+                kb[og + q] = ka[from];
+                sb[og + q] = sa[from];
+                ia += take_a ? 1 : 0;
+                ib += take_a ? 0 : 1;
+            }
+        }
+        __syncthreads();
+        int64_t* tk = ka; ka = kb; kb = tk;
+        uint32_t* ts = sa; sa = sb; sb = ts;
+    }
+    if (ka != s.key) {
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+            s.key[i] = ka[i];
+            s.slot[i] = sa[i];
+        }
+        __syncthreads();
     }
 }
 

@@ -19,8 +19,10 @@
 //   * OSN1 export of each arm's mesh, untimed, from a volume rebuilt after the
 //     timed window.
 //
-// Arms: "hip" (this library's reference-shaped point path) and "hip-lds" (the
-// same path with the allocator's workgroup pre-hash on: its ablation).
+// Arms: "hip" (this library's reference-shaped point path, whose allocator
+// collects a work unit's keys in a workgroup LDS set before the table) and
+// "hip-noset" (the same path with the set off, every wave's keys straight to
+// the table: its ablation).
 // Windows are timed on the host clock around launch + synchronisation of all
 // BATCH volumes (each volume has its own stream, so their launches overlap).
 //
@@ -167,9 +169,9 @@ std::string gpu_state() {
 
 struct Arm {
     const char* name;
-    int32_t lds_set;  // workgroup pre-hash entries, -1 = library default
+    int32_t lds_set;  // workgroup key-set entries, -1 = library default, 0 = off
 };
-const Arm kArms[] = {{"hip", -1}, {"hip-lds", 1024}};
+const Arm kArms[] = {{"hip", -1}, {"hip-noset", 0}};
 constexpr int kNumArms = 2;
 
 OsnTsdfHipVolume* make_volume(float voxel, int32_t pool, const Arm& arm) {
