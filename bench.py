@@ -61,10 +61,11 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counter reads for a 256 MiB device copy (FETCH x 2.0,
 # WRITE x 1.0). Counters cannot run inside this timed benchmark, so the
 # launches they were measured on are named beside them.
-UPDATE_TRAFFIC = {"hbm_bytes_per_launch": 395.4e6, "read": 238.6e6, "write": 156.8e6,
-                  "algorithmic_bytes_per_launch": 432.9e6,
-                  "launches": "k_proj_update_batch<fp16>, 16 scenes, steps 0..29, "
-                              "100.6k touched blocks per launch",
+UPDATE_TRAFFIC = {"hbm_bytes_per_launch": 335.7e6, "read": 239.2e6, "write": 96.5e6,
+                  "algorithmic_bytes_per_launch": 435.0e6,
+                  "launches": "k_proj_update_batch<fp16>, 16 scenes, the timed steps 45..64 "
+                              "(after 40 pre-roll and 5 warmup frames), 101.1k touched blocks "
+                              "per launch; voxels frozen at the weight cap are read, not written",
                   "summary": "profiles/k_proj_update_batch__precise16_640x480.md"}
 
 
