@@ -279,7 +279,8 @@ def alloc_statistics(H, cfg, scenes, untimed, steps, alloc_ms):
     t = alloc_ms * 1e-3
     pixels = len(scenes) * cfg["width"] * cfg["height"]
     lookups = look / steps
-    return {"bound": "atomics (table probes) and VALU (band walk)",
+    return {"bound": "valu (band walk and key handling; the table atomics run far below "
+                     "their ceiling)",
             "pixels_per_launch": pixels,
             "lookups_per_launch": round(lookups, 1),
             "cas_attempts_per_launch": round(cas / steps, 1),
@@ -686,6 +687,10 @@ def main():
             f"touched blocks per step {tb_all.mean():.0f}, algorithmic bytes "
             f"{alg_all.mean() / 1e6:.1f} MB; timed steps {untimed}..{n - 1}: "
             f"{tb.mean():.0f} blocks, {alg.mean() / 1e6:.1f} MB")
+        # The allocate's table statistics over the same timed steps (its
+        # counting and probe variants, separate kernels in a trace).
+        log("allocate statistics " + json.dumps(
+            alloc_statistics(H, cfg, scenes, untimed, args.steps, per[0])))
         return
 
     if args.profile_chain:
