@@ -498,6 +498,26 @@ __device__ inline int32_t n_blocks_of(const VolDev& v, const ExtractArgs& a) {
     return a.n_blocks >= 0 ? a.n_blocks : global(v.ctr)->block_count;
 }
 
+// Blocks [lo, hi) of workgroup w of G (G >= 8, a multiple of 8 as the
+// launchers round it) and its first index and stride in steps of `per`
+// blocks. Workgroups are dealt to the 8 XCDs round-robin by id, so XCD
+// w % 8 takes the (w % 8)-th eighth of the pool, in pool order: a block's +z
+// and +y neighbours (usually a few indices on in key order) and its +x
+// neighbour (a few hundred on) are read by workgroups of the same XCD close
+// in time, from its L2, instead of from HBM by another XCD.
+struct XcdSpan {
+    int32_t lo, hi, first, stride;
+};
+__device__ inline XcdSpan xcd_span(int32_t w, int32_t G, int32_t n, int32_t per) {
+    const int32_t xcd = w % kXcds, mine = (G - xcd + kXcds - 1) / kXcds;
+    XcdSpan sp;
+    sp.lo = (int32_t)((int64_t)n * xcd / kXcds);
+    sp.hi = (int32_t)((int64_t)n * (xcd + 1) / kXcds);
+    sp.first = sp.lo + per * (w / kXcds);
+    sp.stride = per * mine;
+    return sp;
+}
+
 // Exclusive scan of one value per thread across the workgroup; returns the
 // prefix and writes the workgroup total.
 __device__ inline int wg_exclusive_scan(int x, int* s_wave, int& total) {
@@ -521,23 +541,24 @@ __device__ inline int wg_exclusive_scan(int x, int* s_wave, int& total) {
     return before + incl - x;
 }
 
-// Count: one wave per block, lane = row (ly, lz) of 8 cubes; workgroup
-// `first` walks blocks first, first + stride, ...
+// Count: one wave per block, lane = row (ly, lz) of 8 cubes; workgroup w of
+// G walks its XCD's span of the pool (two consecutive blocks at a time for
+// the packed storages).
 template <int STORE>
-__device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t first,
-                             int32_t stride) {
+__device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t w, int32_t G) {
     [[maybe_unused]] __shared__ Cube<STORE> c;
     __shared__ __attribute__((aligned(16))) uint16_t s_table[256];
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
     const int32_t cached = a.nb_cache ? global(v.ctr)->nb_blocks : 0;
-    if (a.clear_flag && first == 0 && lane == 0) *global(a.scan_flag) = 0;
+    if (a.clear_flag && w == 0 && lane == 0) *global(a.scan_flag) = 0;
     if constexpr (STORE != kStoreSums) {
-        // blocks b and b + stride together
+        // blocks b and b + 1 together
         __shared__ Cube<STORE> c2[2];
-        for (int32_t b = first; b < n; b += 2 * stride) {
-            const int32_t b1 = b + stride < n ? b + stride : -1;
+        const XcdSpan sp = xcd_span(w, G, n, 2);
+        for (int32_t b = sp.first; b < sp.hi; b += sp.stride) {
+            const int32_t b1 = b + 1 < sp.hi ? b + 1 : -1;
             load_cube_pair<STORE>(v, b, b1, a.min_weight, c2, a.block_nb, cached);
             uint32_t e[8];
             // two triangle counts (<= 6144 each) in one word, one reduction
@@ -552,7 +573,8 @@ __device__ void count_blocks(const VolDev& v, const ExtractArgs& a, int32_t firs
             __syncthreads();  // the cubes are read before the next pair replaces them
         }
     } else {
-        for (int32_t b = first; b < n; b += stride) {
+        const XcdSpan sp = xcd_span(w, G, n, 1);
+        for (int32_t b = sp.first; b < sp.hi; b += sp.stride) {
             load_cube<STORE, true>(v, b, a.min_weight, c, a.block_nb, cached);
             uint32_t e[8];
             uint32_t n3 = classify_row(c, s_table, lane & 7, lane >> 3, a.iso, e);
@@ -601,16 +623,30 @@ __device__ void scan_chunks(const VolDev& v, const ExtractArgs& a) {
         // chunk sums zero. No fences, no arrival counter.
         if (blockIdx.x != 0) return;
         const int first = threadIdx.x * n_chunks;
+        const int last = min(first + n_chunks, n);
+        // a thread's run is read 8 counts at a time, every load of a batch
+        // in flight before the first is used
+        constexpr int kBatch = 8;
         int sum = 0;
-        for (int k = 0; k < n_chunks; ++k)
-            if (first + k < n) sum += a.block_vcount[first + k];
+        for (int k0 = first; k0 < last; k0 += kBatch) {
+            int c[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) c[j] = k0 + j < last ? global(a.block_vcount)[k0 + j] : 0;
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) sum += c[j];
+        }
         int total;  // <= 64 x 1024 blocks x 18432 vertices < 2^31
         int run = wg_exclusive_scan(sum, s_wave, total);
-        for (int k = 0; k < n_chunks; ++k)
-            if (first + k < n) {
-                a.block_voff[first + k] = run;
-                run += a.block_vcount[first + k];
+        for (int k0 = first; k0 < last; k0 += kBatch) {
+            int c[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) c[j] = k0 + j < last ? global(a.block_vcount)[k0 + j] : 0;
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) {
+                if (k0 + j < last) global(a.block_voff)[k0 + j] = run;
+                run += c[j];
             }
+        }
         if ((int)threadIdx.x < n_chunks) a.chunk_sums[threadIdx.x] = 0;
         if (threadIdx.x == 0) {
             const int32_t out = total > a.cap ? -1 : total;
@@ -666,22 +702,25 @@ __global__ __launch_bounds__(kScanChunk) void k_extract_scan_batch(
     scan_chunks(vols[blockIdx.y], args[blockIdx.y]);
 }
 
-// Emit: one wave per block, lane = row (ly, lz) of 8 cubes. The block's
+// Emit: one wave per block, blocks dealt to workgroups round-robin (the
+// blocks' triangle counts vary along the pool, and an interleaved deal keeps
+// the XCDs evenly loaded where contiguous spans did not), lane = row (ly, lz)
+// of 8 cubes. The block's
 // triangles are listed in LDS in output order (voxel, tetrahedron, half), so
 // triangle j's vertices are 3j.. of the block's run, and emitted one per lane
 // in groups of 64; each group's 192 vertices are staged and copied out as one
 // contiguous run.
 template <int STORE>
-__device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t first, int32_t stride) {
+__device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t w, int32_t G) {
     __shared__ Cube<STORE> c;
     __shared__ __attribute__((aligned(16))) uint16_t s_table[256];
-    __shared__ uint16_t s_slots[kSlotCap + kWaveThreads];  // + one dummy entry per lane
+    __shared__ uint16_t s_slots[kSlotCap + kWaveThreads];
     __shared__ Stage s_stage;
     if (*global(a.count_out) < 0) return;  // overflow: write nothing
     build_table(s_table);
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
-    for (int32_t b = first; b < n; b += stride) {
+    for (int32_t b = w; b < n; b += G) {
         if (global(a.block_vcount)[b] == 0) continue;
         load_cube<STORE, false>(v, b, a.min_weight, c, a.block_nb);
         const int32_t bx = global(v.block_coord)[b * 3 + 0];
@@ -781,10 +820,13 @@ __global__ __launch_bounds__(kWaveThreads, emit_waves_per_simd<STORE>()) void k_
 hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipStream_t s) {
     // one-wave workgroups: one per block when the count is known on the host,
     // otherwise a grid-stride over the pool, four times the chip's worth
-    const int32_t grid_blocks = a.n_blocks >= 0 ? (a.n_blocks > 0 ? a.n_blocks : 1)
-                                                : (v.pool_capacity < 32768 ? v.pool_capacity : 32768);
+    auto xcd_round = [](int32_t g) { return (g + kXcds - 1) / kXcds * kXcds; };
+    const int32_t grid_blocks = xcd_round(
+        a.n_blocks >= 0 ? (a.n_blocks > 0 ? a.n_blocks : 1)
+                        : (v.pool_capacity < 32768 ? v.pool_capacity : 32768));
     // count: two blocks per workgroup for the packed storages
-    const int32_t count_grid = v.storage == kStoreSums ? grid_blocks : (grid_blocks + 1) / 2;
+    const int32_t count_grid =
+        v.storage == kStoreSums ? grid_blocks : xcd_round((grid_blocks + 1) / 2);
     const int32_t n_for_scan = a.n_blocks >= 0 ? a.n_blocks : v.pool_capacity;
     const int32_t n_chunks = (n_for_scan + kScanChunk - 1) / kScanChunk;
     const int32_t scan_grid = n_chunks < 1 ? 1 : (n_chunks < kScanGrid ? n_chunks : kScanGrid);
@@ -814,9 +856,11 @@ hipError_t launch_extract_batch(const VolDev* vols, const ExtractArgs* args, int
                                 int32_t grid_per_scene, bool emit, hipStream_t s) {
     if (n_scenes <= 0 || grid_per_scene <= 0) return hipErrorInvalidValue;
     const uint32_t cap = max_pool_capacity > 0 ? max_pool_capacity : 1;
-    const dim3 grid(cap < (uint32_t)grid_per_scene ? cap : (uint32_t)grid_per_scene, n_scenes);
+    auto xcd_round = [](uint32_t g) { return (g + kXcds - 1) / kXcds * kXcds; };
+    const dim3 grid(xcd_round(cap < (uint32_t)grid_per_scene ? cap : (uint32_t)grid_per_scene),
+                    n_scenes);
     // count: two blocks per workgroup
-    const dim3 count_grid((grid.x + 1) / 2, n_scenes);
+    const dim3 count_grid(xcd_round((grid.x + 1) / 2), n_scenes);
     const uint32_t n_chunks = (cap + kScanChunk - 1) / kScanChunk;
     const dim3 scan_grid(n_chunks < kScanGridBatch ? n_chunks : kScanGridBatch, n_scenes);
     if (!emit) {

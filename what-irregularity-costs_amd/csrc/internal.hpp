@@ -159,6 +159,9 @@ hipError_t launch_extract(const VolDev& v, const ExtractArgs& a, bool emit, hipS
 // each: blockIdx.y is the volume, grid_per_scene one-wave workgroups walk its
 // blocks (fewer when the pools are smaller).
 constexpr int32_t kExtractGridPerScene = 16384;
+// XCDs of the chip: workgroups are dealt to them round-robin by id, each XCD
+// with its own L2.
+constexpr int kXcds = 8;
 hipError_t launch_extract_batch(const VolDev* vols, const ExtractArgs* args, int32_t n_scenes,
                                 int32_t storage, uint32_t max_pool_capacity,
                                 int32_t grid_per_scene, bool emit, hipStream_t s);

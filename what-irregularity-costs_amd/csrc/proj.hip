@@ -400,7 +400,6 @@ __global__ __launch_bounds__(512) void k_proj_update(VolDev v, ProjCam cam, Call
 // contiguous chunk per XCD (workgroups are dealt to the 8 XCDs round-robin by
 // id), so each XCD's L2 serves the depth images of only the few scenes in its
 // chunk.
-constexpr int kXcds = 8;
 static_assert(kMaxBatchScenes <= 64, "one lane per scene");
 
 template <int STORE, bool TAB>
