@@ -7,12 +7,12 @@
   1e-4 (FP32) truncation units, key sets and pool order identical.
 * Large workload, 100 noisy 1280x960 frames (5 mm voxels, trunc 2 cm, table
   2^25), through the batched FP16 pipeline with the mesh extracted inside the
-  captured step into a device buffer sized for the whole triangle soup (about
-  a billion vertices: the noise at the far walls is several voxels), against
+  captured step into a device buffer of 2^30 vertices (the soup is about 316 M
+  vertices: the noise at the far walls is several voxels), against
   the FP64 oracle kernel over the same frames: every observed voxel within
   0.01 truncation units with exact weights, the soups' vertex counts within
   the mesh gate's tolerance, and the mesh gate on four seeded 0.8 m crops of
-  the two soups (the host gate cannot hold a billion-vertex soup).
+  the two soups (the host gate cannot hold soups of this size).
 """
 import numpy as np
 import pytest

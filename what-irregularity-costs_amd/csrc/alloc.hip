@@ -34,6 +34,9 @@ constexpr int kAllocWavesPerSimd = 6;
 constexpr int kWgList = 512;        // LDS touched-list capacity per workgroup
 constexpr int kWgNew = 256;         // LDS new-key-list capacity per workgroup
 constexpr int kMaxSet = 1024;       // LDS key-set capacity
+// Slots a lane probes on its own before an unresolved lookup is handed to the
+// whole wave (64 slots per round).
+constexpr uint32_t kLaneProbes = 4;
 
 enum SrcKind : int { kSrcPoints = 0, kSrcDepth = 1, kSrcCoords = 2 };
 
@@ -197,11 +200,108 @@ __device__ inline uint32_t set_slot(int64_t key, int set_size) {
     return (lo ^ (lo >> 18) ^ (hi >> 4)) & uint32_t(set_size - 1);
 }
 
+// Lookups still unresolved after their lanes' own probes (the lanes of
+// `pend`, each with its next slot and the slots inspected so far), resolved
+// one at a time by the whole wave: a round reads the next 64 slots of the
+// key's probe sequence, one per lane, and takes the first slot holding the
+// key or, before it, the first empty one (a compare-and-swap by the lane that
+// read it; a slot another key took meanwhile continues the scan after it).
+// Compiled only into the kernel that inserts given block coordinates (the
+// allocate-only stress path, whose tables run at high load factors); the
+// walks' tables stay sparse and keep per-lane probing to the probe bound.
+// Every lane of the wave calls it; lane l's result is for its own lookup when
+// l is in `pend`.
+struct CoopProbe {
+    bool found, inserted;
+    uint32_t slot;
+    int32_t idx, stamp;
+    uint32_t steps, n_cas, rounds;
+};
+template <int MODE>
+__device__ inline CoopProbe coop_probe(HashEntry OSN_GLOBAL* table,
+                                                          uint32_t capacity, uint32_t mask,
+                                                          int32_t hash_kind, uint32_t probe_limit,
+                                                          unsigned long long pend, int64_t key,
+                                                          uint32_t slot, uint32_t steps) {
+    const int lane = __lane_id();
+    CoopProbe out{false, false, slot, kIdxPending, 0, steps, 0u, 0u};
+    auto wrap = [&](uint32_t s) {
+        return hash_kind == kHashTeschnerPow2 ? (s & mask) : (s >= capacity ? s - capacity : s);
+    };
+    uint32_t rounds = 0;
+    while (pend) {
+        const int l = __ffsll((long long)pend) - 1;
+        pend &= pend - 1;
+        const int64_t kk = readlane64(key, l);
+        uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)slot, l);   // next slot to inspect
+        uint32_t depth = (uint32_t)__builtin_amdgcn_readlane((int)steps, l);  // slots inspected
+        bool f = false, ins = false;
+        int32_t fidx = kIdxPending, fst = 0;
+        uint32_t fslot = base, n_cas = 0;
+        while (depth < probe_limit) {
+            ++rounds;
+            const uint32_t room = probe_limit - depth;  // <= capacity: a round never wraps twice
+            const bool in = (uint32_t)lane < room;
+            const uint32_t s = wrap(base + (uint32_t)lane);
+            int64_t cur = kEmptyKey;
+            int32_t ci = kIdxPending, cs = 0;
+            if (in) {
+                HashEntry OSN_GLOBAL* ent = table + s;
+                cur = __hip_atomic_load(&ent->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ci = __hip_atomic_load(&ent->block_idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                cs = __hip_atomic_load(&ent->stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const unsigned long long mk = __ballot(in && cur == kk);
+            const unsigned long long me = __ballot(in && cur == kEmptyKey);
+            const int at_k = mk ? __ffsll((long long)mk) - 1 : 64;
+            const int at_e = me ? __ffsll((long long)me) - 1 : 64;
+            if (at_k < at_e) {  // the key, before any empty slot
+                f = true;
+                fslot = (uint32_t)__builtin_amdgcn_readlane((int)s, at_k);
+                fidx = __builtin_amdgcn_readlane(ci, at_k);
+                fst = __builtin_amdgcn_readlane(cs, at_k);
+                depth += (uint32_t)at_k + 1;
+                break;
+            }
+            if (at_e < 64) {  // the first empty slot of the sequence
+                depth += (uint32_t)at_e + 1;
+                fslot = (uint32_t)__builtin_amdgcn_readlane((int)s, at_e);
+                if (MODE == kAllocNoCas) break;
+                ++n_cas;
+                int64_t prev = kEmptyKey;
+                if (lane == at_e)
+                    __hip_atomic_compare_exchange_strong(&table[fslot].key, &prev, kk,
+                                                         __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                prev = readlane64(prev, at_e);
+                if (prev == kEmptyKey) { ins = true; break; }
+                if (prev == kk) { f = true; break; }  // index pending, stamp 0, as a lane's own CAS
+                base = wrap(fslot + 1);  // another key took the slot: go on after it
+                continue;
+            }
+            const uint32_t adv = room < 64u ? room : 64u;
+            depth += adv;
+            base = wrap(base + adv);
+        }
+        if (lane == l) {
+            out.found = f;
+            out.inserted = ins;
+            out.slot = fslot;
+            out.idx = fidx;
+            out.stamp = fst;
+            out.steps = depth;
+            out.n_cas = n_cas;
+        }
+    }
+    out.rounds = rounds;
+    return out;
+}
+
 // One batch of up to 64 distinct keys, one per lane (go = lane holds one),
 // against the global table: find or insert, the touched and new lists, and
 // the per-workgroup statistics.
-template <int MODE, bool TOUCH>
-__device__ void table_keys(const TileCtx& c, bool go, int64_t key) {
+template <int MODE, bool TOUCH, bool COOP = false>
+__device__ __attribute__((always_inline)) void table_keys(const TileCtx& c, bool go, int64_t key) {
     const VolDev& v = *c.v;
     WgShared& s = *c.s;
     bool inserted = false, found = false;
@@ -210,9 +310,17 @@ __device__ void table_keys(const TileCtx& c, bool go, int64_t key) {
     int32_t idx = kIdxPending;
     int32_t stamp = 0;  // the slot's stamp as read with the key
     unsigned long long cas = 0, drops = 0;
+    // Each lane first walks its own key's probe sequence for a few slots; the
+    // few lanes still unresolved after that (deep probes at high load
+    // factors) are then resolved one at a time by the whole wave, 64
+    // consecutive slots per round trip, so a wave no longer runs as many
+    // rounds as its deepest lane has slots to inspect.
+    bool stopped = false;  // a lane that reached an empty slot without inserting (no-CAS mode)
     if (go) {
         slot = home_slot(v, key);
-        for (uint32_t p = 0; p < v.probe_limit; ++p) {
+        const uint32_t own =
+            COOP && v.probe_limit > kLaneProbes ? kLaneProbes : v.probe_limit;
+        for (uint32_t p = 0; p < own; ++p) {
             ++steps;
             // key, index and stamp of the slot in one round trip: no index
             // changes during this phase, and a key inserted during it still
@@ -226,7 +334,7 @@ __device__ void table_keys(const TileCtx& c, bool go, int64_t key) {
                 __hip_atomic_load(&ent->stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (cur == key) { found = true; idx = ci; stamp = cs; break; }
             if (cur == kEmptyKey) {
-                if (MODE == kAllocNoCas) break;
+                if (MODE == kAllocNoCas) { stopped = true; break; }
                 if (MODE == kAllocCountCas) cas += 1;
                 int64_t prev = kEmptyKey;  // the slot's key after the exchange attempt
                 __hip_atomic_compare_exchange_strong(&ent->key, &prev, key, __ATOMIC_RELAXED,
@@ -236,6 +344,29 @@ __device__ void table_keys(const TileCtx& c, bool go, int64_t key) {
             }
             slot = next_slot(v, slot);
         }
+    }
+    uint32_t wave_rounds = 0;  // probe rounds of the whole wave (probe mode)
+    if (MODE == kAllocProbe) {
+        uint32_t mx = steps;
+        for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+        wave_rounds = mx;
+    }
+    const unsigned long long pend =
+        COOP ? __ballot(go && !found && !inserted && !stopped && steps < v.probe_limit) : 0ull;
+    if (COOP && pend) {
+        const CoopProbe r = coop_probe<MODE>(glob(v.table), v.capacity, v.mask, v.hash_kind,
+                                             v.probe_limit, pend, key, slot, steps);
+        if ((pend >> __lane_id()) & 1ull) {
+            found = r.found;
+            inserted = r.inserted;
+            slot = r.slot;
+            if (r.found) { idx = r.idx; stamp = r.stamp; }
+            steps = r.steps;
+            if (MODE == kAllocCountCas) cas += r.n_cas;
+        }
+        wave_rounds += r.rounds;
+    }
+    if (go) {
         if (MODE != kAllocNoCas && !found && !inserted) drops += 1;  // probe bound hit
         if (found && idx == kIdxDropped) drops += 1;
     }
@@ -252,12 +383,12 @@ __device__ void table_keys(const TileCtx& c, bool go, int64_t key) {
     if (MODE == kAllocProbe) {
         const unsigned long long ws = wave_sum_u64(steps);
         const unsigned long long wl = wave_sum_u64(go ? 1ull : 0ull);
-        uint32_t mx = steps;
-        for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
         if (__lane_id() == 0) {
             atomicAdd(&s.lane_steps, ws);
             atomicAdd(&s.lookups, wl);
-            atomicAdd(&s.wave_steps, 64ull * mx);  // the wave runs until its deepest lane is done
+            // the wave's probe rounds: its own-lane rounds, then one per
+            // cooperative scan
+            atomicAdd(&s.wave_steps, 64ull * wave_rounds);
         }
         if (go) {
             const int bin = steps < (uint32_t)kProbeHistBins ? int(steps) - 1 : kProbeHistBins - 1;
@@ -289,10 +420,10 @@ __device__ void table_keys(const TileCtx& c, bool go, int64_t key) {
 // the unit costs one table lookup and the waves' batches of a few keys cost
 // no table round trip. A key the crowded set cannot take after 8 probes goes
 // to the table at once. Without a set every batch goes to the table.
-template <int MODE, bool TOUCH>
-__device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
+template <int MODE, bool TOUCH, bool COOP = false>
+__device__ __attribute__((always_inline)) void process_keys(const TileCtx& c, bool has, int64_t key) {
     if (c.set_size == 0) {
-        table_keys<MODE, TOUCH>(c, has, key);
+        table_keys<MODE, TOUCH, COOP>(c, has, key);
         return;
     }
     WgShared& s = *c.s;
@@ -311,7 +442,7 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
             h = (h + 1) & (c.set_size - 1);
         }
     }
-    if (__ballot(spill)) table_keys<MODE, TOUCH>(c, spill, key);
+    if (__ballot(spill)) table_keys<MODE, TOUCH, COOP>(c, spill, key);
 }
 
 // End of a work unit: every key the set holds goes to the table, and the set
@@ -320,7 +451,7 @@ __device__ void process_keys(const TileCtx& c, bool has, int64_t key) {
 // wave's lanes at the table instead of a few lanes of every wave; a key past
 // the list's capacity goes to the table at once. Every thread of the
 // workgroup calls it.
-template <int MODE, bool TOUCH>
+template <int MODE, bool TOUCH, bool COOP = false>
 __device__ void flush_set(const TileCtx& c) {
     if (c.set_size == 0) return;
     WgShared& s = *c.s;
@@ -337,7 +468,7 @@ __device__ void flush_set(const TileCtx& c) {
             at = __builtin_amdgcn_readlane(at, leader) + __popcll(m & lanemask_lt());
             const bool spill = has && at >= kAllocThreads;
             if (has && !spill) s.dense[at] = key;
-            if (__ballot(spill)) table_keys<MODE, TOUCH>(c, spill, key);
+            if (__ballot(spill)) table_keys<MODE, TOUCH, COOP>(c, spill, key);
         }
         if (i < c.set_size) s.set[i] = kEmptyKey;
     }
@@ -346,7 +477,7 @@ __device__ void flush_set(const TileCtx& c) {
     for (int b0 = (int)(threadIdx.x & ~63); b0 < n; b0 += blockDim.x) {
         const int i = b0 + __lane_id();
         const bool has = i < n;
-        table_keys<MODE, TOUCH>(c, has, has ? s.dense[i] : kEmptyKey);
+        table_keys<MODE, TOUCH, COOP>(c, has, has ? s.dense[i] : kEmptyKey);
     }
     __syncthreads();
     if (threadIdx.x == 0) s.n_dense = 0;
@@ -738,7 +869,7 @@ __device__ void alloc_tile(const TileCtx& c, const PointSrc& src, const int32_t*
         const int64_t key =
             has ? pack_coord(glob(coords)[i * 3 + 0], glob(coords)[i * 3 + 1], glob(coords)[i * 3 + 2])
                 : kEmptyKey;
-        process_keys<MODE, TOUCH>(c, has, key);
+        process_keys<MODE, TOUCH, true>(c, has, key);
         return;
     }
     float px = 0.f, py = 0.f, pz = 0.f;
@@ -804,7 +935,7 @@ __global__ __launch_bounds__(kAllocThreads, kAllocWavesPerSimd) void k_alloc(Vol
     for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         if (SRC == kSrcDepth) alloc_depth_unit<MODE, TOUCH>(c, src, tile, 1);
         else alloc_tile<MODE, TOUCH, SRC>(c, src, coords, tile);
-        flush_set<MODE, TOUCH>(c);
+        flush_set<MODE, TOUCH, SRC == kSrcCoords>(c);
     }
     flush_lists(v, c.lc, s);
     flush_stats<MODE>(v, s);
