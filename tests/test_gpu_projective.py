@@ -421,3 +421,72 @@ def test_storage_errors(H, S):
     d = torch.ones(64, dtype=torch.float32, device="cuda")
     assert r.update_depth(d, fr) == 3             # projective update needs mean storage
     assert r.allocate_depth(None, fr) == 1
+
+
+def test_degenerate_and_ragged_frames_fp64_bit_exact(H, S):
+    """Edge cases of the depth path, bit-exact against the CPU FP64 oracle on
+    one volume, frame after frame: a ragged 31x5 image (neither side a
+    multiple of the 8x4 depth tiles or of the 16x16 walk tiles), the same
+    image with every pixel invalid (0, negative, NaN, +inf: nothing is
+    allocated or updated), with a single valid pixel, a 1x1 image, a 4099x1
+    row (one pixel tall, wider than any tile row), and valid pixels mixed
+    with NaN, +inf and negative ones."""
+    cfg = dict(S.PRECISE, table=1 << 16)
+    _, base = frames_for(H, S, cfg, 31, 3, W=31, Hh=5)
+    seq = []
+    f0, d0, _, _ = base[0]
+    seq.append((f0, d0))
+    bad = d0.clone()
+    bad[0::4] = 0.0
+    bad[1::4] = -1.0
+    bad[2::4] = float("nan")
+    bad[3::4] = float("inf")
+    seq.append((base[1][0], bad))
+    one = torch.zeros_like(d0)
+    one[77] = base[2][1][77]
+    seq.append((base[2][0], one))
+    for W, Hh in ((1, 1), (4099, 1)):
+        _, fr = frames_for(H, S, cfg, 33, 1, W=W, Hh=Hh, noise=False)
+        seq.append((fr[0][0], fr[0][1]))
+    # valid pixels with invalid ones among them: voxels of the blocks the
+    # valid ones touch project onto NaN (skipped), +inf (observed as free
+    # space, x = 1) and negative (skipped) depths
+    mixed = base[2][1].clone()
+    mixed[0::5] = float("nan")
+    mixed[1::7] = float("inf")
+    mixed[2::11] = -2.0
+    seq.append((base[2][0], mixed))
+    gv = H.HipVolume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 32.0, storage=H.STORE_F64,
+                     table_slots=cfg["table"])
+    for f, d in seq:
+        assert gv.integrate_depth(d, f) == 0
+    gv.synchronize()
+    ov = B.Volume(cfg["voxel"], cfg["table"] // 2, cfg["trunc"], 0.0)
+    po = B.ProjOracle(cfg["voxel"], cfg["trunc"], 32, cfg["table"] // 2)
+    counts = []
+    for f, d in seq:
+        pts = host_points(H, f, d)
+        cam = tuple(f.t_wc[:])
+        before = ov.block_count
+        if pts.size:
+            ov.allocate(pts, cam)
+            touched = ov.touched_keys(pts, cam)
+        else:
+            touched = np.zeros(0, np.int64)
+        counts.append((pts.size // 3, ov.block_count - before, touched.size))
+        Rwc64 = np.array(f.R_wc64[:]).reshape(3, 3)
+        Rwc32 = np.array(f.R_wc[:], np.float32).reshape(3, 3)
+        po.integrate(touched, d.cpu().numpy().reshape(f.height, f.width), f.fx, f.fy, f.cx, f.cy,
+                     Rwc64.T, np.array(f.t_wc64[:]), Rwc32.T, np.array(f.t_wc[:], np.float32))
+    print("points / new blocks / touched per frame:", counts)
+    assert counts[1] == (0, 0, 0)  # every pixel invalid
+    assert counts[2][0] == 1 and counts[2][2] >= 1  # one pixel: its band's blocks
+    assert counts[3][0] == 1 and counts[4][0] > 1000 and counts[5][2] > 0
+    keys = gv.keys()
+    assert np.array_equal(np.sort(keys), np.sort(ov.keys()))
+    assert gv.drop_count == 0
+    t64, w64 = gv.read_voxels_f64()
+    for i, k in enumerate(keys):
+        ot, ow = po.block(int(k))
+        assert np.array_equal(w64[i], ow.astype(np.float64)), f"weights differ in {k}"
+        assert np.array_equal(t64[i], ot), f"tsdf differs in {k}"
