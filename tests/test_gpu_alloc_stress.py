@@ -109,3 +109,33 @@ def test_extreme_key_sorts_last():
     want = np.unique(stress.pack_keys(c))
     assert want[-1] == np.iinfo(np.int64).max
     assert np.array_equal(v.keys(), want)
+
+
+@pytest.mark.parametrize("box,offset", [((16, 16, 16), (0, 0, 0)), ((1, 1, 4096), (-7, 3, -2000)),
+                                        ((64, 64, 1), (100, -200, 5)), ((32, 32, 64), (-16, -16, -32)),
+                                        ((40, 40, 40), (1, 2, 3))])
+def test_small_box_calls_pool_order(box, offset):
+    """Calls whose new keys fill a small box of block coordinates (ranked
+    through a bitmap of at most 2^16 cells: 16^3, a line, a plane, the
+    32x32x64 limit; 40^3 exceeds it and is sorted), with more than 256 new
+    keys per call: pool order is the per-call key order, over consecutive
+    calls into the same volume."""
+    import stress
+    import osn_tsdf_hip as H
+    rng = np.random.default_rng(11)
+    v = H.HipVolume(0.01, 1 << 15, -1.0, 0.0, storage=H.STORE_F16, table_slots=1 << 16,
+                    hash_kind=1)
+    seen = set()
+    expect = []
+    lo = np.array(offset, np.int32)
+    ext = np.array(box, np.int32)
+    for n in (700, 3000, 2500):
+        c = (lo + (rng.random((n, 3)) * ext).astype(np.int32)).astype(np.int32)
+        assert v.insert_blocks(torch.from_numpy(c.ravel()).cuda()) == 0
+        k = stress.pack_keys(c)
+        new = np.unique(k[~np.isin(k, np.fromiter(seen, np.int64, len(seen)))])
+        expect.extend(new.tolist())
+        seen.update(new.tolist())
+        assert v.block_count == len(expect)
+        assert np.array_equal(v.keys(), np.array(expect, np.int64))
+    assert len(expect) > 1000

@@ -1015,7 +1015,120 @@ struct AssignLds {
     int32_t flag;
     int32_t corank[2];
     unsigned long long drops;
+    uint32_t box_min[3], box_max[3];  // the tile's key fields' ranges (bitmap ranking)
 };
+
+constexpr int kRankBitsMax = 16;  // bitmap ranking over boxes of at most 2^16 cells
+
+// Rank a tile's keys through a bitmap when they span a small box of block
+// coordinates: keys of one call are distinct, and a key's three 21-bit fields
+// order it as the integer does, so with each field offset by the box's low
+// corner and the three concatenated (x high, z low) into a cell index of at
+// most kRankBitsMax bits, a key's rank is the number of occupied cells below
+// its own: one bit set per key, a prefix count over the bitmap's words, no
+// comparisons. The bitmap and its word prefixes reuse key2. False, with
+// nothing moved, when the box is larger (the caller then sorts). Every
+// thread of a 1024-thread workgroup calls it; cnt <= kSortTile.
+__device__ bool bitmap_rank(AssignLds& s, int cnt) {
+    static_assert(kSortTile <= 4 * 1024, "at most 4 keys per thread");
+    static_assert((2 << kRankBitsMax) / 32 * 4 <= (int)sizeof(s.key2), "bitmap and prefixes fit");
+    constexpr uint64_t kM = (uint64_t(1) << kCoordBits) - 1;
+    auto field = [&](uint64_t k, int q) {
+        return uint32_t((k >> (kCoordBits * (2 - q))) & kM);
+    };
+    if (threadIdx.x < 3) {
+        s.box_min[threadIdx.x] = ~0u;
+        s.box_max[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    uint32_t mn[3] = {~0u, ~0u, ~0u}, mx[3] = {0u, 0u, 0u};
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const uint64_t k = (uint64_t)s.key[i];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            mn[q] = min(mn[q], field(k, q));
+            mx[q] = max(mx[q], field(k, q));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[q] = min(mn[q], (uint32_t)__shfl_xor((int)mn[q], o));
+            mx[q] = max(mx[q], (uint32_t)__shfl_xor((int)mx[q], o));
+        }
+        if (__lane_id() == 0) {
+            atomicMin(&s.box_min[q], mn[q]);
+            atomicMax(&s.box_max[q], mx[q]);
+        }
+    }
+    __syncthreads();
+    uint32_t lo[3], bits[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        lo[q] = s.box_min[q];
+        const uint32_t r = s.box_max[q] - lo[q];
+        bits[q] = r ? 32u - (uint32_t)__clz(r) : 0u;
+    }
+    const uint32_t total = bits[0] + bits[1] + bits[2];
+    if (total > (uint32_t)kRankBitsMax) return false;  // the same answer in every thread
+    uint32_t* bm = reinterpret_cast<uint32_t*>(s.key2);
+    uint32_t* pre = bm + (1 << kRankBitsMax) / 32;
+    const int n_words = ((1 << total) + 31) / 32;
+    for (int w = threadIdx.x; w < n_words; w += blockDim.x) bm[w] = 0u;
+    __syncthreads();
+    auto cell = [&](uint64_t k) {
+        return ((field(k, 0) - lo[0]) << (bits[1] + bits[2])) |
+               ((field(k, 1) - lo[1]) << bits[2]) | (field(k, 2) - lo[2]);
+    };
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const uint32_t c = cell((uint64_t)s.key[i]);
+        atomicOr(&bm[c >> 5], 1u << (c & 31u));
+    }
+    __syncthreads();
+    // exclusive prefix of the words' popcounts: a thread takes words 2t, 2t + 1
+    const int w0 = 2 * (int)threadIdx.x;
+    const uint32_t p0 = w0 < n_words ? (uint32_t)__popc(bm[w0]) : 0u;
+    const uint32_t p1 = w0 + 1 < n_words ? (uint32_t)__popc(bm[w0 + 1]) : 0u;
+    uint32_t incl = p0 + p1;
+    const int lane = __lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)incl, o);
+        if (lane >= o) incl += t;
+    }
+    uint32_t* wave_tot = s.slot2;
+    if (lane == 63) wave_tot[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int wv = 0; wv < (int)(threadIdx.x >> 6); ++wv) before += wave_tot[wv];
+    const uint32_t ex = before + incl - (p0 + p1);
+    if (w0 < n_words) pre[w0] = ex;
+    if (w0 + 1 < n_words) pre[w0 + 1] = ex + p0;
+    __syncthreads();
+    // ranks in registers, then every key moves to its place
+    int64_t k[4];
+    uint32_t sl[4], rank[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int i = (int)threadIdx.x + j * (int)blockDim.x;
+        if (i < cnt) {
+            k[j] = s.key[i];
+            sl[j] = s.slot[i];
+            const uint32_t c = cell((uint64_t)k[j]);
+            rank[j] = pre[c >> 5] + (uint32_t)__popc(bm[c >> 5] & ((1u << (c & 31u)) - 1u));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int i = (int)threadIdx.x + j * (int)blockDim.x;
+        if (i < cnt) {
+            s.key[rank[j]] = k[j];
+            s.slot[rank[j]] = sl[j];
+        }
+    }
+    __syncthreads();
+    return true;
+}
 
 __device__ inline bool entry_less(int64_t ka, uint32_t sa, int64_t kb, uint32_t sb) {
     return ka < kb || (ka == kb && sa < sb);
@@ -1052,6 +1165,7 @@ __device__ void lds_sort(AssignLds& s, int cnt) {
         __syncthreads();
         return;
     }
+    if (bitmap_rank(s, cnt)) return;
     int P = 512;
     while (P < cnt) P <<= 1;
     for (int i = cnt + threadIdx.x; i < P; i += blockDim.x) {
