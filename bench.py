@@ -297,6 +297,51 @@ def alloc_statistics(H, cfg, scenes, untimed, steps, alloc_ms):
             "wave_over_lane_steps": round(wave / max(lane, 1), 3)}
 
 
+def measured_ceilings(touched_blocks):
+    """Two ceilings measured on this GPU: a plain device-to-device copy of
+    1 GiB (bytes read + written per second) and the update's own memory
+    pattern with no arithmetic: `touched_blocks` random 2 KiB blocks of a
+    2 GiB pool read and written back, the next block's load issued before the
+    current block's store (what-irregularity-costs_amd/rmw_ceiling, its
+    "unordered" rmw-next line)."""
+    import subprocess
+    import torch
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    for _ in range(2):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    reps = 10
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    copy_gbs = 2 * src.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    out = {"stream_copy_gbs": round(copy_gbs, 1),
+           "stream_copy": "torch copy_ of 1 GiB on the device, 10 repetitions, read + write"}
+    exe = os.path.join(ROOT, "what-irregularity-costs_amd", "rmw_ceiling")
+    if not os.path.exists(exe):
+        out["pattern_error"] = f"{exe} is missing: build it (what-irregularity-costs_amd/build.py)"
+        log(out["pattern_error"])
+        return out
+    r = subprocess.run([exe, str(int(touched_blocks)), str(1 << 20), "16384"],
+                       capture_output=True, text=True, timeout=120)
+    line = next((l for l in r.stdout.splitlines() if l.startswith("unordered")), None)
+    if r.returncode or line is None:
+        out["pattern_error"] = f"rmw_ceiling failed ({r.returncode}): {r.stderr.strip()[-200:]}"
+        log(out["pattern_error"])
+        return out
+    us = float(line.split("rmw-next")[1].split("us")[0])
+    out["pattern_ceiling_us"] = us
+    out["pattern"] = (f"{int(touched_blocks)} random 2 KiB blocks read and written back, next "
+                      "block's load before the current store, no arithmetic (rmw_ceiling)")
+    return out
+
+
 CHAIN_VERTS = 1 << 25  # per-scene mesh buffer (vertices); noisy 1 cm fields make dense soups
 
 
@@ -774,6 +819,11 @@ def main():
     log("allocate statistics")
     alloc = alloc_statistics(H, cfg, scenes, untimed, args.steps,
                              float(ms16[0] / max(calls16[0], 1)))
+    log("measured ceilings")
+    ceil = measured_ceilings(tb16.mean())
+    if "pattern_ceiling_us" in ceil:
+        ceil["update_over_pattern_ceiling"] = round(float(upd_ms16) * 1e3 / ceil["pattern_ceiling_us"], 3)
+    ceil["frac_of_stream_copy"] = round(float(achieved16) / ceil["stream_copy_gbs"], 4)
 
     workload = None
     if not args.no_workload:
@@ -848,7 +898,8 @@ def main():
                      "traffic_measured_on": UPDATE_TRAFFIC,
                      "kernel": "k_proj_update_batch<fp16>",
                      "avg_launch_ms": round(float(upd_ms16), 5),
-                     "bytes_per_launch": round(float(bytes16.mean()), 0)},
+                     "bytes_per_launch": round(float(bytes16.mean()), 0),
+                     "measured_ceilings": ceil},
         "alloc_roofline": alloc,
         "cpu_baseline": cpu,
         "autotune": tune_log,
