@@ -1104,6 +1104,9 @@ __device__ bool bitmap_rank(AssignLds& s, int cnt) {
     if (w0 < n_words) pre[w0] = ex;
     if (w0 + 1 < n_words) pre[w0 + 1] = ex + p0;
     __syncthreads();
+    // one occupied cell per key, as distinct keys give; anything else is
+    // left to the comparison sort (key2 is its scratch too)
+    if (pre[n_words - 1] + (uint32_t)__popc(bm[n_words - 1]) != (uint32_t)cnt) return false;
     // ranks in registers, then every key moves to its place
     int64_t k[4];
     uint32_t sl[4], rank[4];
