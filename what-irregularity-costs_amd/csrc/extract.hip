@@ -274,17 +274,20 @@ __device__ void load_cube_pair(const VolDev& v, int32_t b0, int32_t b1, float mi
     __syncthreads();
 }
 
-// The 6 tetrahedra (corner ids, 4 bits each, lowest first):
-// {0,5,1,7} {0,1,3,7} {0,3,2,7} {0,6,4,7} {0,4,5,7} {0,2,6,7}.
-// Kept as immediates: an array indexed by a lane-varying t would live in
-// scratch memory.
-__device__ inline uint32_t tet_corners(int t) {
-    return t < 4 ? (uint32_t)(0x7460723073107150ull >> (16 * t)) & 0xffffu
-                 : (0x76207540u >> (16 * (t - 4))) & 0xffffu;
-}
+// The 6 tetrahedra of a cube, corner ids as the reference lists them
+// (cpp/src/marching_tets.cpp:37-41); every one holds the 0-7 diagonal.
+constexpr int kTetList[6][4] = {{0, 5, 1, 7}, {0, 1, 3, 7}, {0, 3, 2, 7},
+                                {0, 6, 4, 7}, {0, 4, 5, 7}, {0, 2, 6, 7}};
 
 // The same tetrahedra as corner bit sets, one byte each, lowest first.
-constexpr unsigned long long kTetBits = 0xC5B1D18D8BA3ull;
+constexpr unsigned long long make_tet_bits() {
+    unsigned long long b = 0;
+    for (int t = 0; t < 6; ++t)
+        for (int j = 0; j < 4; ++j) b |= 1ull << (8 * t + kTetList[t][j]);
+    return b;
+}
+constexpr unsigned long long kTetBits = make_tet_bits();
+static_assert(kTetBits == 0xC5B1D18D8BA3ull, "tetrahedra bit sets");
 
 // Classification of a cube from its corner mask (bit k set when corner k is
 // below iso): per tetrahedron t its triangles in bits 2t..2t+1 (0 none, 1 one,
@@ -414,58 +417,160 @@ __device__ inline float staged(const Stage& st, int i, int c) {
     return c < 3 ? st.pos[i * 3 + c] : st.nor[i - i % 3 + (c - 3)];
 }
 
-// Corner k's rank in the lexicographic order of positions (x, then y, then
-// z): the order in which an edge's two endpoints are taken.
-__device__ inline int lex_key(int k) { return ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1); }
+// Every tetrahedron holds the cube's main diagonal, and its corners ordered by
+// position (x, then y, then z: the order in which the reference takes an
+// edge's endpoints, cpp/src/marching_tets.cpp:59-80) form a monotone path
+// 0 -> e_a1 -> e_a1 + e_a2 -> 7 that sets one axis per step. So an edge's
+// endpoints in position order are its corners in path order (rank 0-3), and
+// along the edge an axis either stays or goes from the lower to the upper
+// coordinate. The tables below are built by the compiler from the tetrahedra
+// and the reference's case switch (cpp/src/marching_tets.cpp:157-183).
+constexpr int corner_lds_offset(int k) { return (k & 1) + 9 * ((k >> 1) & 1) + 81 * (k >> 2); }
+constexpr int lex_rank_key(int k) { return ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1); }
 
-// Per crossing code (bit j set when the tetrahedron's j-th corner is below
-// iso) the roles (a, b, c, d) as positions 0-3 among its corners, 2 bits
-// each, a lowest; 0 for the codes without a crossing (0 and 15). Codes with
-// one or three corners below iso cut triangle (ab, ac, ad), codes with two a
-// quad split into (ac, ad, bd) and (ac, bd, bc).
-constexpr unsigned long long kRolesLo = 0x93C9D8D2E4E1E400ull;  // codes 0-7
-constexpr unsigned long long kRolesHi = 0x00E4E14ED28D9C93ull;  // codes 8-15
+struct EmitTables {
+    // (flip, code, half) -> the triangle's three edges (vertex order), each
+    // as its endpoint ranks r0 < r1 in bits 4j..4j+3 (r0 low); bit 15 set
+    // for a triangle that exists. code bit r: the rank-r corner is below iso;
+    // flip: the tetrahedron lists its rank-2 corner second.
+    uint16_t tri[64];
+    // per tetrahedron: cube-array offsets of its rank-1 and rank-2 corners
+    // (bits 0-6, 7-13), flip (bit 14), the rank at which each axis becomes
+    // the upper coordinate (x: bits 16-17, y: 18-19, z: 20-21)
+    uint32_t tet[8];
+};
+
+constexpr EmitTables make_emit_tables() {
+    EmitTables T{};
+    for (int flip = 0; flip < 2; ++flip) {
+        // rank of the tetrahedron's listed corner j
+        const int rank[4] = {0, flip ? 2 : 1, flip ? 1 : 2, 3};
+        for (int code = 0; code < 16; ++code) {
+            int listed = 0;  // the crossing code in listed-corner order
+            for (int j = 0; j < 4; ++j) listed |= ((code >> rank[j]) & 1) << j;
+            // roles (a, b, c, d) as listed corners; single: triangle (ab, ac,
+            // ad); quad: (ac, ad, bd) then (ac, bd, bc)
+            int a = -1, b = 0, c = 0, d = 0;
+            bool quad = false;
+            switch (listed) {
+                case 0x01: case 0x0E: a = 0; b = 1; c = 2; d = 3; break;
+                case 0x02: case 0x0D: a = 1; b = 0; c = 2; d = 3; break;
+                case 0x04: case 0x0B: a = 2; b = 0; c = 1; d = 3; break;
+                case 0x08: case 0x07: a = 3; b = 0; c = 1; d = 2; break;
+                case 0x03: a = 0; b = 1; c = 2; d = 3; quad = true; break;
+                case 0x0C: a = 2; b = 3; c = 0; d = 1; quad = true; break;
+                case 0x05: a = 0; b = 2; c = 1; d = 3; quad = true; break;
+                case 0x0A: a = 1; b = 3; c = 0; d = 2; quad = true; break;
+                case 0x06: a = 1; b = 2; c = 0; d = 3; quad = true; break;
+                case 0x09: a = 0; b = 3; c = 1; d = 2; quad = true; break;
+                default: break;
+            }
+            for (int half = 0; half < 2; ++half) {
+                uint16_t e = 0;
+                if (a >= 0 && (quad || half == 0)) {
+                    int ed[3][2] = {{a, b}, {a, c}, {a, d}};
+                    if (quad && half == 0) {
+                        ed[0][0] = a; ed[0][1] = c; ed[1][0] = a; ed[1][1] = d;
+                        ed[2][0] = b; ed[2][1] = d;
+                    } else if (quad) {
+                        ed[0][0] = a; ed[0][1] = c; ed[1][0] = b; ed[1][1] = d;
+                        ed[2][0] = b; ed[2][1] = c;
+                    }
+                    e = 0x8000u;
+                    for (int j = 0; j < 3; ++j) {
+                        int r0 = rank[ed[j][0]], r1 = rank[ed[j][1]];
+                        if (r1 < r0) { const int s = r0; r0 = r1; r1 = s; }
+                        e = (uint16_t)(e | ((r0 | (r1 << 2)) << (4 * j)));
+                    }
+                }
+                T.tri[(flip * 16 + code) * 2 + half] = e;
+            }
+        }
+    }
+    for (int t = 0; t < 6; ++t) {
+        const int A = kTetList[t][1], B = kTetList[t][2];
+        const bool flip = lex_rank_key(B) < lex_rank_key(A);
+        const int L1 = flip ? B : A, L2 = flip ? A : B;
+        uint32_t w = (uint32_t)corner_lds_offset(L1) | ((uint32_t)corner_lds_offset(L2) << 7) |
+                     ((flip ? 1u : 0u) << 14);
+        for (int ax = 0; ax < 3; ++ax) {
+            const int s = ((L1 >> ax) & 1) ? 1 : (((L2 >> ax) & 1) ? 2 : 3);
+            w |= (uint32_t)s << (16 + 2 * ax);
+        }
+        T.tet[t] = w;
+    }
+    return T;
+}
+static_assert(sizeof(EmitTables) % 4 == 0 && sizeof(EmitTables) / 4 <= 64, "one dword per lane");
+__constant__ EmitTables kEmitTables = make_emit_tables();
+
+__device__ inline void load_emit_tables(EmitTables& t) {
+    if (threadIdx.x < sizeof(EmitTables) / 4)
+        reinterpret_cast<uint32_t*>(&t)[threadIdx.x] =
+            reinterpret_cast<const uint32_t*>(&kEmitTables)[threadIdx.x];
+}
 
 // Triangle `half` (0, or 1 for a quad's second) of tetrahedron t of voxel
 // (lx, ly, lz)'s cube, staged from vertex index at, colours written from
-// at_rgb. Every lane takes the same path: three edge interpolations and one
-// triangle. Corner positions, values and colours are formed per corner id,
-// never held in an array indexed by a lane-varying value.
+// at_rgb. Every lane takes the same path: its tetrahedron's four corner
+// values read once in rank order, one table entry for the case, three edge
+// interpolations and one triangle; values, positions and colours are picked
+// by rank with selects (no array indexed by a lane-varying value).
+// Selects by a lane-varying rank, taking values (a conditional over lvalues
+// would select their addresses, and the compiler keeps addressed locals in
+// scratch memory).
+__device__ inline float pick_lo(int r, float a, float b, float c) {  // r in 0-2
+    return r == 0 ? a : (r == 1 ? b : c);
+}
+__device__ inline float pick_hi(int r, float b, float c, float d) {  // r in 1-3
+    return r == 3 ? d : (r == 2 ? c : b);
+}
+__device__ inline int pick_lo_i(int r, int a, int b, int c) { return r == 0 ? a : (r == 1 ? b : c); }
+__device__ inline int pick_hi_i(int r, int b, int c, int d) { return r == 3 ? d : (r == 2 ? c : b); }
+__device__ inline float axis_point(int r0, int r1, int s, float a0, float a1, float d, float w) {
+    const float p0 = r0 >= s ? a1 : a0;
+    return p0 + w * ((r0 < s && r1 >= s) ? d : 0.0f);
+}
+
 template <int STORE>
-__device__ void emit_slot(const Cube<STORE>& c, const VolDev& v, int32_t bx, int32_t by,
-                          int32_t bz, int lx, int ly, int lz, float iso, int t, bool half,
-                          Stage& st, int at, uint8_t* rgb, int64_t at_rgb) {
+__device__ void emit_slot(const Cube<STORE>& c, const EmitTables& et, const VolDev& v,
+                          int32_t bx, int32_t by, int32_t bz, int lx, int ly, int lz, float iso,
+                          int t, bool half, Stage& st, int at, uint8_t* rgb, int64_t at_rgb) {
     constexpr bool kCol = STORE == kStoreSums;
+    const uint32_t tw = et.tet[t];
+    const int o1 = (int)(tw & 127u), o2 = (int)((tw >> 7) & 127u);
+    const int cb = cube_index(lx, ly, lz);
+    const float V0 = corner(c.val[cb]), V1 = corner(c.val[cb + o1]);
+    const float V2 = corner(c.val[cb + o2]), V3 = corner(c.val[cb + 91]);
+    const uint32_t code = (V0 < iso ? 1u : 0u) | (V1 < iso ? 2u : 0u) | (V2 < iso ? 4u : 0u) |
+                          (V3 < iso ? 8u : 0u);
+    const uint32_t ent = et.tri[(((tw >> 10) & 16u) | code) * 2u + (half ? 1u : 0u)];
+    if (!(ent & 0x8000u)) return;
     const int32_t vx = bx * 8 + lx, vy = by * 8 + ly, vz = bz * 8 + lz;
-    // each axis' two corner coordinates (offset 0 and 1)
+    // each axis' two corner coordinates, as the reference forms them
     const float x0 = ((float)vx + 0.5f) * v.voxel, x1 = ((float)(vx + 1) + 0.5f) * v.voxel;
     const float y0 = ((float)vy + 0.5f) * v.voxel, y1 = ((float)(vy + 1) + 0.5f) * v.voxel;
     const float z0 = ((float)vz + 0.5f) * v.voxel, z1 = ((float)(vz + 1) + 0.5f) * v.voxel;
-    // corner k at cube index base + dx + 9 dy + 81 dz (24-bit multiply-adds)
-    const int cbase = cube_index(lx, ly, lz);
-    auto ci = [&](int k) {
-        return cbase + (k & 1) + (int)__umul24((unsigned)((k >> 1) & 1), 9u) +
-               (int)__umul24((unsigned)(k >> 2), 81u);
-    };
-    auto val = [&](int k) { return corner(c.val[ci(k)]); };
-    // the point of edge (ka, kb) where the mean tsdf crosses iso, endpoints in
-    // position order (cpp/src/marching_tets.cpp:37-80)
-    auto interp = [&](int ka, int kb) {
-        const bool sw = lex_key(kb) < lex_key(ka);
-        const int k0 = sw ? kb : ka, k1 = sw ? ka : kb;
-        const float v0 = val(k0), v1 = val(k1);
+    // an edge that moves along an axis goes from its lower to its upper
+    // coordinate: p1 - p0 is then this difference, otherwise 0
+    const float dx = x1 - x0, dy = y1 - y0, dz = z1 - z0;
+    const int sx = (int)((tw >> 16) & 3u), sy = (int)((tw >> 18) & 3u), sz = (int)((tw >> 20) & 3u);
+    auto interp = [&](int j) {
+        const int r0 = (int)((ent >> (4 * j)) & 3u), r1 = (int)((ent >> (4 * j + 2)) & 3u);
+        // r0 < r1: r0 is 0-2, r1 is 1-3
+        const float v0 = pick_lo(r0, V0, V1, V2);
+        const float v1 = pick_hi(r1, V1, V2, V3);
         const float den = v1 - v0;
         float w = fabsf(den) < 1e-12f ? 0.5f : (iso - v0) / den;
         w = fminf(fmaxf(w, 0.0f), 1.0f);
-        const float px0 = (k0 & 1) ? x1 : x0, px1 = (k1 & 1) ? x1 : x0;
-        const float py0 = (k0 & 2) ? y1 : y0, py1 = (k1 & 2) ? y1 : y0;
-        const float pz0 = (k0 & 4) ? z1 : z0, pz1 = (k1 & 4) ? z1 : z0;
+        // p0 + w * (p1 - p0) per axis; w * 0 adds a zero, as in the reference
         V6 o;
-        o.p[0] = px0 + w * (px1 - px0);
-        o.p[1] = py0 + w * (py1 - py0);
-        o.p[2] = pz0 + w * (pz1 - pz0);
+        o.p[0] = axis_point(r0, r1, sx, x0, x1, dx, w);
+        o.p[1] = axis_point(r0, r1, sy, y0, y1, dy, w);
+        o.p[2] = axis_point(r0, r1, sz, z0, z1, dz, w);
         if constexpr (kCol) {
-            const int i0 = ci(k0), i1 = ci(k1);
+            const int i0 = cb + pick_lo_i(r0, 0, o1, o2);
+            const int i1 = cb + pick_hi_i(r1, o1, o2, 91);
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 const float c0 = c.col[q][i0], c1 = c.col[q][i1];
@@ -476,21 +581,7 @@ __device__ void emit_slot(const Cube<STORE>& c, const VolDev& v, int32_t bx, int
         }
         return o;
     };
-    const uint32_t tc = tet_corners(t);
-    const int code = (val((int)(tc & 15u)) < iso ? 1 : 0) | (val((int)((tc >> 4) & 15u)) < iso ? 2 : 0) |
-                     (val((int)((tc >> 8) & 15u)) < iso ? 4 : 0) | (val((int)(tc >> 12)) < iso ? 8 : 0);
-    const uint32_t roles =
-        (uint32_t)((code < 8 ? kRolesLo >> (8 * code) : kRolesHi >> (8 * (code - 8))) & 0xffu);
-    if (roles == 0u) return;
-    auto role = [&](int j) { return (int)((tc >> (4u * ((roles >> (2 * j)) & 3u))) & 15u); };
-    const int a = role(0), b = role(1), cc = role(2), d = role(3);
-    const bool quad = __popc((uint32_t)code) == 2;
-    // triangle (ab, ac, ad); quad halves (ac, ad, bd) and (ac, bd, bc)
-    // endpoints chosen first, so every lane runs the same three interpolations
-    const bool second = quad && half;
-    const V6 e0 = interp(a, quad ? cc : b);
-    const V6 e1 = interp(second ? b : a, second ? d : (quad ? d : cc));
-    const V6 e2 = interp(quad ? b : a, second ? cc : d);
+    const V6 e0 = interp(0), e1 = interp(1), e2 = interp(2);
     write_tri(e0, e1, e2, st, at, rgb, at_rgb);
 }
 
@@ -716,8 +807,10 @@ __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t w, in
     __shared__ __attribute__((aligned(16))) uint16_t s_table[256];
     __shared__ uint16_t s_slots[kSlotCap + kWaveThreads];
     __shared__ Stage s_stage;
+    __shared__ EmitTables s_et;
     if (*global(a.count_out) < 0) return;  // overflow: write nothing
-    build_table(s_table);
+    load_emit_tables(s_et);
+    build_table(s_table);  // (its barrier also covers s_et)
     const int lane = threadIdx.x;
     const int32_t n = n_blocks_of(v, a);
     for (int32_t b = w; b < n; b += G) {
@@ -777,7 +870,7 @@ __device__ void emit_blocks(const VolDev& v, const ExtractArgs& a, int32_t w, in
                 if (i < n_pass) {
                     const uint32_t it = s_slots[i];
                     const int vi = (int)(it & 511u), t = (int)((it >> 9) & 7u);
-                    emit_slot(c, v, bx, by, bz, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, t,
+                    emit_slot(c, s_et, v, bx, by, bz, vi & 7, (vi >> 3) & 7, vi >> 6, a.iso, t,
                               ((it >> 12) & 1u) != 0u, s_stage, 3 * lane, a.rgb, v_lo + 3 * lane);
                 }
                 const int n_verts = 3 * cnt;
