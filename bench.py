@@ -33,7 +33,9 @@ Reported:
                  -> extract), batched fp16 against plain fp32;
   batched_workload  24 independent scenes of uneven size (320x240, 640x480,
                  1280x720, tables 2^16..2^22), frame counts scaled to a
-                 bounded run, tuned on frames before the timed ones;
+                 bounded run, tuned on frames before the timed ones; timed
+                 again with every scene's mesh extracted in the graph after
+                 every step;
   precision      max / RMS |dtsdf| against the FP64 oracle kernel after the
                  full 500-frame trajectory of one scene, in truncation units
                  and in voxels, and the meshes through the mesh gate;
@@ -482,6 +484,26 @@ def run_batched_workload(H, S, seed=4, n_scenes=24, scale=0.15, tune_frames=12):
     b.run(steps - 1)
     ms, calls = b.read_timing()
     close_all(b, vols)
+    # The same schedule with every scene's mesh extracted inside the graph
+    # after every step (allocate -> assign -> update -> count/scan/emit for all
+    # scenes), each scene into its own device buffer.
+    bufs = [torch.empty(CHAIN_VERTS * 6, dtype=torch.float32, device="cuda") for _ in range(n)]
+    counts = torch.zeros(n, dtype=torch.int32, device="cuda")
+    b, vols = batch()
+    if b.set_extract(bufs, [CHAIN_VERTS] * n, counts, 0.5, 0.0, every=1):
+        raise RuntimeError("set_extract failed")
+    b.run(1, use_graph=True)
+    b.synchronize()
+    t0 = time.perf_counter()
+    if b.run(steps - 1, use_graph=True):
+        raise RuntimeError("batch run failed")
+    b.synchronize()
+    dt_x = time.perf_counter() - t0
+    mesh_verts = counts.cpu().tolist()
+    close_all(b, vols)
+    if min(mesh_verts) < 0:
+        raise RuntimeError(f"a mesh overflowed its buffer: {mesh_verts}")
+    del bufs
     del rend
     torch.cuda.empty_cache()
     sizes = {}
@@ -499,6 +521,13 @@ def run_batched_workload(H, S, seed=4, n_scenes=24, scale=0.15, tune_frames=12):
             "stage_ms_per_step": {s: round(float(ms[i] / max(calls[i], 1)), 5)
                                   for i, s in enumerate(H.STAGES[:3])},
             "blocks_allocated": int(sum(blocks)), "scenes_with_drops": int(sum(d > 0 for d in drops)),
+            "with_extraction": {
+                "stages": "allocate + assign + update + extract (device mesh of every scene "
+                          "after every step), one graph per step",
+                "frames_per_s": round(timed_frames / dt_x, 1),
+                "ms_per_step": round(dt_x * 1e3 / (steps - 1), 4),
+                "last_mesh_vertices": int(sum(mesh_verts)),
+                "largest_mesh_vertices": int(max(mesh_verts))},
             "autotune": {"chosen": best, "trials": len(tlog["trials"]),
                          "seconds": tlog["seconds"],
                          "tuned_on": f"{tune_frames} frames per scene before the timed ones"}}

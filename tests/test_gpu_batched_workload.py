@@ -11,7 +11,11 @@ Pins:
   * the first step's allocated key set of every scene equals the oracle's
     allocation over the same back-projected points (cpp/src/volume.cpp:147-198);
   * FP16 storage stays within 0.01 trunc of the FP64 oracle kernel on two
-    scenes of different image size, with exact weights.
+    scenes of different image size, with exact weights;
+  * the mesh the batch extracts inside its graph after every step (one
+    count, scan and emit launch for all scenes, each into its own device
+    buffer) is, after the last step, byte-identical to the device extraction
+    of the same volume through the single-volume call, for every scene.
 Scenes idle (NULL depth) after their last frame, and a fifth of them also
 idle every fourth step before it.
 """
@@ -81,25 +85,33 @@ def state(v):
     return k, t.view(np.uint32), w.view(np.uint32)
 
 
+MESH_VERTS = 1 << 24  # per-scene device mesh buffer (vertices)
+
+
 @pytest.fixture(scope="module")
 def batched(H, workload):
-    """The batch after its first step (key sets kept) and after every step."""
+    """The batch after its first step (key sets kept) and after every step,
+    with every scene's mesh extracted inside the graph after each step."""
     cfgs, depths, frames, n_steps = workload
+    n = len(cfgs)
     vols = [volume(H, c, H.STORE_F16) for c in cfgs]
     b = H.Batch(vols)
     assert b.set_schedule(depths, frames) == 0
+    meshes = [torch.empty(MESH_VERTS * 6, dtype=torch.float32, device="cuda") for _ in range(n)]
+    counts = torch.zeros(n, dtype=torch.int32, device="cuda")
+    assert b.set_extract(meshes, [MESH_VERTS] * n, counts, 0.5, 0.0, every=1) == 0
     assert b.run(1, use_graph=True) == 0
     b.synchronize()
     first = [(v.keys(), v.drop_count) for v in vols]
     assert b.run(n_steps - 1, use_graph=True) == 0
     b.synchronize()
     b.close()
-    return vols, first
+    return vols, first, (meshes, counts.cpu().numpy())
 
 
 def test_first_step_key_sets_match_oracle(H, workload, batched):
     cfgs, depths, frames, _ = workload
-    _, first = batched
+    _, first, _ = batched
     checked = 0
     for s, c in enumerate(cfgs):
         d, fr = depths[0][s], frames[0][s]
@@ -122,7 +134,7 @@ def test_first_step_key_sets_match_oracle(H, workload, batched):
 
 def test_batched_state_equals_per_scene(H, workload, batched):
     cfgs, depths, frames, n_steps = workload
-    vols, _ = batched
+    vols, _, _ = batched
     drops = 0
     for s, c in enumerate(cfgs):
         solo = volume(H, c, H.STORE_F16)
@@ -142,7 +154,7 @@ def test_batched_state_equals_per_scene(H, workload, batched):
 
 def test_fp16_within_gate_on_two_scenes(H, workload, batched):
     cfgs, depths, frames, n_steps = workload
-    vols, _ = batched
+    vols, _, _ = batched
     # the scene with the most frames of the largest and of the smallest size
     pick = []
     for size in ((1280, 720), (320, 240)):
@@ -168,3 +180,21 @@ def test_fp16_within_gate_on_two_scenes(H, workload, batched):
         assert obs.sum() > 100000
         assert err.max() <= 0.01
         ref.close()
+
+
+def test_in_graph_meshes_equal_single_volume_extraction(H, workload, batched):
+    cfgs, _, _, _ = workload
+    vols, _, (meshes, counts) = batched
+    out = torch.empty(MESH_VERTS * 6, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    total = 0
+    for s, v in enumerate(vols):
+        n = int(counts[s])
+        assert n > 0, f"scene {s}: {n}"
+        assert v.extract_device(out, None, MESH_VERTS, 0.5, 0.0, cnt) == 0
+        v.synchronize()
+        assert int(cnt.item()) == n, f"scene {s}"
+        assert torch.equal(meshes[s][: 6 * n].view(torch.int32), out[: 6 * n].view(torch.int32)), \
+            f"scene {s} ({cfgs[s]['width']}x{cfgs[s]['height']})"
+        total += n
+    print(f"{len(vols)} in-graph meshes, {total} vertices, byte-identical")
