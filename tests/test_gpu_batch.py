@@ -153,6 +153,40 @@ def test_graph_replay_matches_eager_every_step(H, S):
         assert same(state(v1), state(v2))
 
 
+@pytest.mark.parametrize("every", [0, 1, 5])
+def test_multi_step_graphs_match_eager(H, S, every):
+    """A graph call of 19 steps replays runs of 8 alike steps (all or none
+    extracting) from one 8-step graph and the rest from one-step graphs:
+    with no extraction (two 8-step graphs, three single steps), extraction
+    after every step (the same, each step meshing every scene) and after
+    every 5th (no run of 8 is alike: single steps only). States and the last
+    meshes equal eager runs of the same schedule."""
+    n_steps = 19
+    depths, frames = make_inputs(H, S, n_steps, seed0=150)
+    caps = [1 << 24] * 3
+    runs = []
+    for use_graph in (False, True):
+        vs = volumes(H, 2)
+        b = H.Batch(vs)
+        assert b.set_schedule(depths, frames) == 0
+        bufs = [torch.empty(c * 6, dtype=torch.float32, device="cuda") for c in caps]
+        counts = torch.full((3,), -7, dtype=torch.int32, device="cuda")
+        if every:
+            assert b.set_extract(bufs, caps, counts, 0.5, 0.0, every=every) == 0
+        assert b.run(n_steps, use_graph=use_graph) == 0
+        b.synchronize()
+        runs.append((vs, bufs, counts.cpu().numpy()))
+        b.close()
+    (ve, be, ce), (vg, bg, cg) = runs
+    for v1, v2 in zip(ve, vg):
+        assert v1.block_count == v2.block_count > 0
+        assert same(state(v1), state(v2))
+    if every:
+        assert np.array_equal(ce, cg) and (ce > 0).all()
+        for s in range(3):
+            assert _same_mesh(be[s], bg[s], ce[s]), f"scene {s}"
+
+
 def test_member_reads_wait_for_the_batch(H, S):
     """Counters and voxels of a batched volume read straight after an
     asynchronous batch step (no explicit synchronisation) are that step's: the

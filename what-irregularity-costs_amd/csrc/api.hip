@@ -996,6 +996,13 @@ struct OsnTsdfHipBatch {
     double ms[kStages] = {0, 0, 0, 0};
     int64_t stage_calls[kStages] = {0, 0, 0, 0};
 
+    // Runs of kGraphSteps steps alike in extraction are replayed from one
+    // graph of that many steps (index: with extraction), so the per-launch
+    // cost of a graph is paid once per run rather than once per step.
+    static constexpr int kGraphSteps = 8;
+    hipGraph_t graph_m[2] = {nullptr, nullptr};
+    hipGraphExec_t exec_m[2] = {nullptr, nullptr};
+
     void drop_graph() {
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
@@ -1005,6 +1012,19 @@ struct OsnTsdfHipBatch {
         graph = nullptr;
         exec_x = nullptr;
         graph_x = nullptr;
+        for (int i = 0; i < 2; ++i) {
+            if (exec_m[i]) hipGraphExecDestroy(exec_m[i]);
+            if (graph_m[i]) hipGraphDestroy(graph_m[i]);
+            exec_m[i] = nullptr;
+            graph_m[i] = nullptr;
+        }
+    }
+    // Whether steps first .. first + kGraphSteps - 1 all extract (x) or none does.
+    bool uniform_run(int32_t first, bool& x) const {
+        x = extracts_after(first);
+        for (int j = 1; j < kGraphSteps; ++j)
+            if (extracts_after(first + j) != x) return false;
+        return true;
     }
     bool extracts_after(int32_t step) const {
         return extract_every > 0 && (step + 1) % extract_every == 0;
@@ -1028,15 +1048,19 @@ struct OsnTsdfHipBatch {
     }
     // Capture one step (and the extraction after it when `with_extract`) as
     // a graph whose kernels read the step number from d_step.
-    bool capture(bool with_extract, hipGraph_t& g_out, hipGraphExec_t& e_out) {
+    bool capture(bool with_extract, hipGraph_t& g_out, hipGraphExec_t& e_out, int steps = 1) {
         const bool t = timing;
         timing = false;
         if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             timing = t;
             return false;
         }
-        hipError_t err = enqueue_step(dev(0, true));
-        if (err == hipSuccess && with_extract) err = enqueue_extract();
+        // every step reads its index from the device counter the update advances
+        hipError_t err = hipSuccess;
+        for (int i = 0; i < steps && err == hipSuccess; ++i) {
+            err = enqueue_step(dev(0, true));
+            if (err == hipSuccess && with_extract) err = enqueue_extract();
+        }
         hipGraph_t g = nullptr;
         const hipError_t end = hipStreamEndCapture(stream, &g);
         timing = t;
@@ -1231,11 +1255,29 @@ int32_t osn_tsdf_hip_batch_run(OsnTsdfHipBatch* b, int32_t n_steps, int32_t use_
             b->drop_graph();
             return 2;
         }
+        // the runs of kGraphSteps alike steps this call replays: one graph each
+        for (int i = 0; i + OsnTsdfHipBatch::kGraphSteps <= n_steps;) {
+            bool x = false;
+            if (!b->uniform_run(first + i, x)) { ++i; continue; }
+            if (!b->exec_m[x] &&
+                !b->capture(x, b->graph_m[x], b->exec_m[x], OsnTsdfHipBatch::kGraphSteps)) {
+                b->drop_graph();
+                return 2;
+            }
+            i += OsnTsdfHipBatch::kGraphSteps;
+        }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (b->timing) { hipEventCreate(&e0); hipEventRecord(e0, b->stream); }
-        for (int i = 0; i < n_steps; ++i) {
-            hipGraphExec_t x = b->extracts_after(first + i) ? b->exec_x : b->exec;
-            if (hipGraphLaunch(x, b->stream) != hipSuccess) return 2;
+        for (int i = 0; i < n_steps;) {
+            bool x = false;
+            if (i + OsnTsdfHipBatch::kGraphSteps <= n_steps && b->uniform_run(first + i, x)) {
+                if (hipGraphLaunch(b->exec_m[x], b->stream) != hipSuccess) return 2;
+                i += OsnTsdfHipBatch::kGraphSteps;
+                continue;
+            }
+            hipGraphExec_t g = b->extracts_after(first + i) ? b->exec_x : b->exec;
+            if (hipGraphLaunch(g, b->stream) != hipSuccess) return 2;
+            ++i;
         }
         if (b->timing) {
             hipEventCreate(&e1);
