@@ -187,6 +187,32 @@ def test_multi_step_graphs_match_eager(H, S, every):
             assert _same_mesh(be[s], bg[s], ce[s]), f"scene {s}"
 
 
+def test_mixed_graph_and_eager_runs(H, S):
+    """Graph runs read their step index from a device counter that their
+    update kernels advance; eager runs in between leave it behind, and a
+    graph run after one must set it again: 4 graph, 2 eager, 9 graph (an
+    8-step graph and a single step), 1 graph, 1 eager steps equal 17 eager
+    steps."""
+    n_steps = 17
+    depths, frames = make_inputs(H, S, n_steps, seed0=170)
+    ref = volumes(H, 2)
+    b0 = H.Batch(ref)
+    assert b0.set_schedule(depths, frames) == 0
+    assert b0.run(n_steps) == 0
+    b0.synchronize()
+    vs = volumes(H, 2)
+    b = H.Batch(vs)
+    assert b.set_schedule(depths, frames) == 0
+    for k, g in ((4, True), (2, False), (9, True), (1, True), (1, False)):
+        assert b.run(k, use_graph=g) == 0
+    b.synchronize()
+    for v1, v2 in zip(ref, vs):
+        assert v1.block_count == v2.block_count > 0
+        assert same(state(v1), state(v2))
+    b0.close()
+    b.close()
+
+
 def test_member_reads_wait_for_the_batch(H, S):
     """Counters and voxels of a batched volume read straight after an
     asynchronous batch step (no explicit synchronisation) are that step's: the

@@ -973,6 +973,9 @@ struct OsnTsdfHipBatch {
     UpdParams* d_upd = nullptr;
     int32_t n_steps = 0;
     int32_t cursor = 0;
+    // the device step counter equals cursor (a graph run left it there); an
+    // eager run or a new schedule leaves it stale
+    bool step_on_device = false;
     int32_t total_tiles = 0;
     uint32_t max_capacity = 0;
     uint32_t max_pool_capacity = 0;
@@ -1231,6 +1234,7 @@ int32_t osn_tsdf_hip_batch_set_schedule(OsnTsdfHipBatch* b, int32_t n_steps,
         return 2;
     b->n_steps = n_steps;
     b->cursor = 0;
+    b->step_on_device = false;
     b->total_tiles = prefix[n];
     b->depth_sub = sub;
     return 0;
@@ -1240,15 +1244,20 @@ int32_t osn_tsdf_hip_batch_run(OsnTsdfHipBatch* b, int32_t n_steps, int32_t use_
     if (!b || n_steps <= 0 || !b->d_steps) return 1;
     if (b->cursor + n_steps > b->n_steps) return 1;
     if (!use_graph) {
+        b->step_on_device = false;
         for (int i = 0; i < n_steps; ++i) {
             if (b->enqueue_step(b->dev(b->cursor + i, false)) != hipSuccess) return 2;
             if (b->extracts_after(b->cursor + i) && b->enqueue_extract() != hipSuccess) return 2;
         }
     } else {
         const int32_t first = b->cursor;
-        if (hipMemcpyAsync(b->d_step, &first, sizeof(int32_t), hipMemcpyHostToDevice, b->stream) ||
-            hipStreamSynchronize(b->stream))
+        // the graphs' steps read their index from the device counter, which
+        // their update kernels advance: set it unless a graph run left it here
+        if (!b->step_on_device &&
+            (hipMemcpyAsync(b->d_step, &first, sizeof(int32_t), hipMemcpyHostToDevice, b->stream) ||
+             hipStreamSynchronize(b->stream)))
             return 2;
+        b->step_on_device = false;
         if (!b->exec && !b->capture(false, b->graph, b->exec)) { b->drop_graph(); return 2; }
         if (b->extract_every > 0 && !b->exec_x &&
             !b->capture(true, b->graph_x, b->exec_x)) {
@@ -1284,6 +1293,7 @@ int32_t osn_tsdf_hip_batch_run(OsnTsdfHipBatch* b, int32_t n_steps, int32_t use_
             hipEventRecord(e1, b->stream);
             b->ev_pending.push_back({0, {e0, e1}});
         }
+        b->step_on_device = true;
     }
     b->cursor += n_steps;
     for (size_t s = 0; s < b->vols.size(); ++s) b->vols[s]->calls = b->base_calls[s] + b->cursor;
