@@ -411,6 +411,55 @@ def test_extract_after_every_frame_and_reset(H, S):
         assert v.reset() == 0
 
 
+@pytest.mark.parametrize("storage", ["f16", "f32"])
+def test_extraction_sequence_matches_oracle(H, S, storage):
+    """Extractions of one volume reuse the neighbour indices earlier calls
+    found (kept in the volume's scratch). Meshes through a sequence where
+    anything kept from an earlier call could go stale -- an update after an
+    extraction, a repeated update_depth of one allocation, blocks allocated
+    next to old ones, other min_weight and iso and back with no update in
+    between, a reset -- equal the oracle extractor over the same voxel
+    state."""
+    st = H.STORE_F16 if storage == "f16" else H.STORE_F32
+    cfg = dict(S.SMALL)
+    _, fr = frames_for(H, S, cfg, 7, 5)
+    v = H.HipVolume(cfg["voxel"], 1 << 14, cfg["trunc"], 32.0, storage=st)
+
+    def check(min_w, iso, what):
+        host = v.extract(min_w, iso, cap=4 << 20)
+        keys = v.keys()
+        t, w = v.read_voxels()
+        ov = B.Volume(cfg["voxel"], len(keys) + 8, cfg["trunc"], 0.0)
+        for i, k in enumerate(keys):
+            ov.load_block(int(k), t[i], w[i], 1)
+        ref = ov.extract(min_w, iso, cap=4 << 20)
+        assert host is not None and len(host) == len(ref) > 0, what
+        assert np.array_equal(ref, host), what
+
+    for rnd in range(2):
+        f, d, _, _ = fr[0]
+        assert v.integrate_depth(d, f) == 0
+        check(0.5, 0.0, "first frame")
+        check(0.5, 0.0, "no change")
+        for i, (f, d, _, _) in enumerate(fr[1:3]):
+            assert v.integrate_depth(d, f) == 0
+            check(0.5, 0.0, f"frame {i + 1}")
+        # one allocation, two updates, a count between them
+        f, d, _, _ = fr[3]
+        assert v.allocate_depth(d, f) == 0 and v.update_depth(d, f) == 0
+        check(0.5, 0.0, "allocate + update")
+        assert v.update_depth(d, f) == 0
+        check(0.5, 0.0, "the same frame's update again")
+        # other parameters (no update between), then back
+        check(0.5, 0.1, "iso 0.1")
+        check(2.5, 0.1, "min weight 2.5")
+        check(0.5, 0.0, "back to the first parameters")
+        f, d, _, _ = fr[4]
+        assert v.integrate_depth(d, f) == 0
+        check(0.5, 0.0, f"round {rnd} last frame")
+        assert v.reset() == 0
+
+
 def test_storage_errors(H, S):
     v = H.HipVolume(0.02, 1024, -1.0, 32.0, storage=H.STORE_F16)
     pts = torch.zeros(30, dtype=torch.float32, device="cuda")
