@@ -54,9 +54,10 @@ def test_sphere_keys_and_sums_bit_parity(H):
 # Weight caps: the reference skips a visit once the voxel's weight has reached
 # the cap (cpp/src/volume.cpp:283-289, types.hpp:99-101); its sequential CPU
 # arm therefore ends every voxel at min(cap, visits) for an integer cap, and
-# ceil(cap) for a fractional one. The workgroup sums are claimed against the
-# global weight, so the device ends at the same counts whatever the order of
-# its workgroups. Three integrates of the sphere: most surface voxels pass
+# ceil(cap) for a fractional one. The device takes a call's visits of a voxel
+# against the weight the voxel has (all at once, or per workgroup where they
+# are added directly), so it ends at the same counts whatever the order of its
+# workgroups. Three integrates of the sphere: most surface voxels pass
 # cap 8 within one call, many pass 32.5 by the third.
 @pytest.mark.parametrize("cap", [8.0, 32.5])
 def test_scatter_weight_cap_counts_exact(H, cap):
@@ -91,6 +92,45 @@ def test_scatter_weight_cap_counts_exact(H, cap):
     print(f"cap {cap}: {n_sat} saturated voxels, {n_free} below the cap; |mean| max {m.max():.3f}")
     assert n_sat > 10000 and n_free > 1000
     assert m.max() <= 1.0 + 1e-6  # a convex weighting of observations in [-1, 1]
+
+
+# Dense clouds: the 1.28M-point sphere gives most blocks more than 64
+# workgroup segments in one call, so the visit pass adds part of their visits
+# directly; 2.2M points take two chunks of the two-pass update in one call.
+# Weights must still equal the sequential oracle's, capped or not. Near the
+# poles a voxel collects thousands of visits, where two FP32 sums of the same
+# n terms in different orders may differ by up to 2 (n - 1) 2^-24 sum|terms|
+# (sum|terms| <= n = the weight), so the tolerance grows to that bound.
+def sum_tol(w):
+    w = np.maximum(w.astype(np.float64), 1.0)
+    return np.maximum(1e-5 * w, 2.0 * 2.0 ** -24 * w * w)
+
+
+@pytest.mark.parametrize("a,b,cap", [(800, 1600, 0.0), (1100, 2000, 0.0), (1100, 2000, 8.0)])
+def test_scatter_dense_clouds_exact(H, a, b, cap):
+    pts = B.sphere_points(0.5, a, b)
+    gv = H.HipVolume(0.01, 1 << 14, -1.0, cap)
+    assert gv.integrate(dev(pts), (0, 0, 0)) == 0
+    ov = B.Volume(0.01, 1 << 14, -1.0, cap)
+    ov.integrate(pts, (0, 0, 0))
+    ou = B.Volume(0.01, 1 << 14, -1.0, 0.0) if cap else ov
+    if cap:
+        ou.integrate(pts, (0, 0, 0))
+    gk = gv.keys()
+    assert np.array_equal(np.sort(gk), np.sort(ov.keys()))
+    t, w = gv.read_voxels()
+    n_vis = 0
+    for i, k in enumerate(gk):
+        ow = ov.block_voxels(int(k))[1]
+        ut, uw = ou.block_voxels(int(k))[:2]
+        assert np.array_equal(w[i], ow), f"weights differ in block {k}"
+        free = uw <= (np.ceil(cap) if cap else np.inf)
+        err = np.abs(t[i][free].astype(np.float64) - ut[free])
+        assert np.all(err <= sum_tol(uw[free])), f"tsdf sums differ in block {k}"
+        n_vis += int(uw.sum())
+    n_pts = len(pts) // 3
+    print(f"{n_pts} points, {n_vis} visits, {len(gk)} blocks")
+    assert n_vis > 8 * n_pts
 
 
 def test_sphere_analytic_gate_and_known_answer(H):

@@ -1504,19 +1504,38 @@ __global__ __launch_bounds__(1024) void k_assign_batch(BatchDev b) {
 
 // ---- reference-semantics scatter update (volume.cpp:200-299) ---------------
 
-// The workgroup's visits are summed in LDS per voxel first, then flushed with
-// one set of global atomics per distinct voxel: neighbouring points of a
-// workgroup hit the same voxels many times over, and the global float atomics
-// are what the reference's update spends its time on. Weights are exact
-// integer counts either way; a tsdf sum changes only in summation order,
-// which is arbitrary in the reference too (its atomics race). The colour
-// sums are 128 * count, as the reference's repeated 128s add exactly.
-constexpr int kScatterSet = 2048;  // LDS voxel slots per workgroup (>= 256 x 9 visits / 1.1)
+// Every point walks its 2 * steps + 1 samples along the camera ray; a sample
+// inside the truncation band adds one visit (weight 1, the clamped
+// normalised distance) to its voxel. Weights are exact integer counts in any
+// order; a tsdf sum changes only in summation order, which is arbitrary in
+// the reference too (its atomics race). The colour sums are 128 * count, as
+// the reference's repeated 128s add exactly.
+//
+// Float atomics at device scope are performed at the memory side, serially
+// per address and at a few tens of G/s for the whole chip: a single pass
+// that flushes a workgroup's distinct voxels with five of them each spends
+// most of its time there. Hence two passes over the scratch that
+// ScatterScratch describes: the visit pass groups the workgroup's distinct
+// voxels by block and writes them as entries, and the apply pass adds each
+// touched block's totals with plain coalesced loads and stores, each block
+// owned by one workgroup.
 
-struct ScatterLds {
-    unsigned long long key[kScatterSet];  // voxel index in the pool, ~0 = empty
-    float w[kScatterSet];
-    float t[kScatterSet];
+static_assert(kScatterBlocks == kScatterThreads, "one block slot per thread");
+static_assert(kScatterBlocks == 256, "the block-slot hash takes the top 8 bits");
+static_assert(kBlockVoxels == 512 && kScatterSet <= 4096,
+              "a grouped entry packs voxel (9 bits), block slot (8) and rank (12)");
+
+template <typename Key>
+struct ScatterVisitLds {
+    // voxel index in the pool, ~0 = empty; once grouped by block, the voxel
+    // in its block | block slot << 9 | rank among the block's entries << 17
+    Key key[kScatterSet];
+    uint32_t w[kScatterSet];         // visits
+    float t[kScatterSet];            // sum of the visits' normalised distances
+    uint32_t bkey[kScatterBlocks];   // pool block, ~0 = empty
+    uint32_t bcnt[kScatterBlocks];   // the block's entries, then its first entry
+    uint32_t wsum[kScatterThreads / 64];
+    uint8_t direct[kScatterBlocks];  // the block's segments are full: add directly
 };
 
 __device__ inline void scatter_global(const VolDev& v, size_t idx, float w, float t) {
@@ -1528,23 +1547,30 @@ __device__ inline void scatter_global(const VolDev& v, size_t idx, float w, floa
     unsafeAtomicAdd(&v.b[idx], c);
 }
 
-// Capped volumes: the workgroup's w visits of a voxel (sum t) are added only
-// as far as the cap allows. The weight is claimed with a compare-and-swap
-// loop, so concurrent flushes of one voxel are serialised and the voxel ends
-// at exactly min(cap, weight before the call + its visits), rounded up to
-// whole visits (a visit is taken while weight < cap, volume.cpp:289): the
-// count the reference's sequential CPU arm reaches. When only part of the w
-// visits fit, their sum is taken in proportion (t * take / w), a convex
-// weighting of the same observations; which visits the reference keeps at
-// the cap depends on its thread timing anyway.
+// Weight caps: the reference skips a visit once the voxel's weight has
+// reached the cap (volume.cpp:283-289), so its sequential CPU arm ends a
+// voxel at min(cap, weight before the call + its visits), rounded up to
+// whole visits. A group of w visits (sum t) takes take = min(w, ceil(cap -
+// weight)) of them against the weight it finds; the same count results
+// whatever the order in which groups arrive. When only part of the w visits
+// fit, their sum is taken in proportion (t * take / w), a convex weighting of
+// the same observations; which visits the reference keeps at the cap depends
+// on its thread timing anyway.
+__device__ inline float cap_take(float cur, float w, float cap) {
+    return cur < cap ? fminf(w, ceilf(cap - cur)) : 0.0f;
+}
+
+// One group added with global atomics, concurrent with other groups of the
+// same voxel: the weight is claimed with a compare-and-swap loop, so the
+// claims of one voxel are serialised.
 __device__ inline void scatter_global_capped(const VolDev& v, size_t idx, float w, float t) {
     unsigned int* wp = reinterpret_cast<unsigned int*>(&v.weight[idx]);
     unsigned int old = __atomic_load_n(wp, __ATOMIC_RELAXED);
     float take;
     for (;;) {
         const float cur = __uint_as_float(old);
-        if (!(cur < v.weight_cap)) return;
-        take = fminf(w, ceilf(v.weight_cap - cur));
+        take = cap_take(cur, w, v.weight_cap);
+        if (take <= 0.0f) return;
         const unsigned int seen = atomicCAS(wp, old, __float_as_uint(cur + take));
         if (seen == old) break;
         old = seen;
@@ -1556,15 +1582,31 @@ __device__ inline void scatter_global_capped(const VolDev& v, size_t idx, float 
     unsafeAtomicAdd(&v.b[idx], c);
 }
 
-__global__ __launch_bounds__(256) void k_scatter_update(VolDev v, PointSrc src) {
-    __shared__ ScatterLds s;
-    for (int e = threadIdx.x; e < kScatterSet; e += blockDim.x) {
-        s.key[e] = ~0ull;
-        s.w[e] = 0.0f;
+__device__ inline void scatter_direct(const VolDev& v, size_t idx, float w, float t) {
+    if (v.weight_cap > 0.0f) scatter_global_capped(v, idx, w, t);
+    else scatter_global(v, idx, w, t);
+}
+
+template <typename Key>
+__global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, PointSrc src,
+                                                                   ScatterScratch sc) {
+    __shared__ ScatterVisitLds<Key> s;
+    constexpr Key kEmpty = ~Key(0);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kScatterSet; e += kScatterThreads) {
+        s.key[e] = kEmpty;
+        s.w[e] = 0u;
         s.t[e] = 0.0f;
     }
+    s.bkey[tid] = ~0u;
+    s.bcnt[tid] = 0u;
+    // the next call's touched list starts empty (the apply pass of the call
+    // before this one has read it)
+    if (blockIdx.x == 0 && tid < kScatterLists)
+        sc.touched_count[((sc.parity ^ 1) * kScatterLists + tid) * kScatterCtrStride] = 0;
     __syncthreads();
-    const int i = blockIdx.x * 256 + threadIdx.x;
+
+    const int i = blockIdx.x * kScatterThreads + tid;
     float px = 0.f, py = 0.f, pz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, d2 = 0.f, dist = 0.f;
     bool ok = i < src.n;
     if (ok) {
@@ -1596,67 +1638,157 @@ __global__ __launch_bounds__(256) void k_scatter_update(VolDev v, PointSrc src) 
             const int32_t bx = floor_div8(vx), by = floor_div8(vy), bz = floor_div8(vz);
             const int64_t key = pack_coord(bx, by, bz);
             if (key != last_key) { last_key = key; last_bi = find_index(v, key); }
-            if (last_bi < 0) continue;
+            if (last_bi < 0 || last_bi >= v.pool_capacity) continue;
             const float sdf_n = fminf(fmaxf(sdf * inv_trunc, -1.0f), 1.0f);
-            const size_t idx = (size_t)last_bi * kBlockVoxels +
-                               (size_t)(((vz & 7) * 8 + (vy & 7)) * 8 + (vx & 7));
-            // the reference's racy cap check, against the global weight
-            // (volume.cpp:283-289)
-            const bool capped = v.weight_cap > 0.0f;
-            const float g = capped ? glob(v.weight)[idx] : 0.0f;
-            if (capped && g >= v.weight_cap) continue;
-            uint32_t h = uint32_t((uint64_t(idx) * 0x9E3779B97F4A7C15ull) >> 40) & (kScatterSet - 1);
+            const Key idx = (Key)last_bi * kBlockVoxels +
+                            (Key)(((vz & 7) * 8 + (vy & 7)) * 8 + (vx & 7));
+            // multiply-shift onto [0, kScatterSet), which need not be a power of two
+            uint32_t h = uint32_t((uint64_t(uint32_t((uint64_t(idx) * 0x9E3779B97F4A7C15ull) >> 32)) *
+                                   kScatterSet) >> 32);
             bool placed = false;
             for (int p = 0; p < 16; ++p) {
-                const unsigned long long prev = atomicCAS(&s.key[h], ~0ull, (unsigned long long)idx);
-                if (prev == ~0ull || prev == (unsigned long long)idx) {
-                    if (capped) {
-                        // The workgroup's own visits are counted against the
-                        // cap too: a visit claims one unit of the LDS weight
-                        // only while the global weight it read plus the
-                        // workgroup's claims so far stay below the cap, so a
-                        // workgroup carries at most the visits the cap still
-                        // admits to the flush, which then claims them from
-                        // the global weight (scatter_global_capped).
-                        unsigned int* wb = reinterpret_cast<unsigned int*>(&s.w[h]);
-                        unsigned int old = __atomic_load_n(wb, __ATOMIC_RELAXED);
-                        bool claimed = false;
-                        while (g + __uint_as_float(old) < v.weight_cap) {
-                            const unsigned int seen =
-                                atomicCAS(wb, old, __float_as_uint(__uint_as_float(old) + 1.0f));
-                            if (seen == old) {
-                                claimed = true;
-                                break;
-                            }
-                            old = seen;
-                        }
-                        if (claimed) atomicAdd(&s.t[h], sdf_n);
-                    } else {
-                        atomicAdd(&s.w[h], 1.0f);
-                        atomicAdd(&s.t[h], sdf_n);
-                    }
+                const Key prev = atomicCAS(&s.key[h], kEmpty, idx);
+                if (prev == kEmpty || prev == idx) {
+                    atomicAdd(&s.w[h], 1u);
+                    atomicAdd(&s.t[h], sdf_n);
                     placed = true;
                     break;
                 }
-                h = (h + 1) & (kScatterSet - 1);
+                h = h + 1 == kScatterSet ? 0u : h + 1;
             }
-            if (!placed) {  // LDS table crowded
-                if (capped)
-                    scatter_global_capped(v, idx, 1.0f, sdf_n);
-                else
-                    scatter_global(v, idx, 1.0f, sdf_n);
-            }
+            if (!placed) scatter_direct(v, (size_t)idx, 1.0f, sdf_n);  // LDS set crowded
         }
     }
     __syncthreads();
+
+    // the distinct voxels by block: a slot per block, a rank per entry
+    for (int e = tid; e < kScatterSet; e += kScatterThreads) {
+        const Key k = s.key[e];
+        if (k == kEmpty) continue;
+        const uint32_t bi = (uint32_t)(k / kBlockVoxels);
+        uint32_t h = (bi * 0x9E3779B1u) >> 24;  // kScatterBlocks = 256 slots
+        int found = -1;
+        for (int p = 0; p < kScatterBlocks; ++p) {
+            const uint32_t prev = atomicCAS(&s.bkey[h], ~0u, bi);
+            if (prev == ~0u || prev == bi) {
+                found = (int)h;
+                break;
+            }
+            h = (h + 1) & (kScatterBlocks - 1);
+        }
+        if (found < 0) {  // more distinct blocks than slots
+            scatter_direct(v, (size_t)k, (float)s.w[e], s.t[e]);
+            s.key[e] = kEmpty;
+            continue;
+        }
+        const uint32_t rank = atomicAdd(&s.bcnt[found], 1u);
+        s.key[e] = (Key)((uint32_t)(k % kBlockVoxels) | ((uint32_t)found << 9) | (rank << 17));
+    }
+    __syncthreads();
+
+    // thread j owns block slot j: its first entry in the region (exclusive
+    // scan of the counts) and the block's segment for this workgroup
+    const uint32_t c = s.bcnt[tid];
+    const int lane = tid & 63;
+    uint32_t incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s.wsum[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t first = incl - c;
+    for (int w = 0; w < (tid >> 6); ++w) first += s.wsum[w];
+    const uint32_t region = blockIdx.x * (uint32_t)sc.region;
+    uint8_t direct = 0;
+    if (c > 0) {
+        const uint32_t bi = s.bkey[tid];
+        const int seg = atomicAdd(&sc.seg_count[bi], 1);
+        if (seg == 0) {
+            const uint32_t j = bi % kScatterLists;
+            const int at = atomicAdd(
+                &sc.touched_count[(sc.parity * kScatterLists + j) * kScatterCtrStride], 1);
+            sc.touched[(size_t)j * sc.list_cap + at] = (int32_t)bi;
+        }
+        if (seg < kScatterSegs)
+            sc.segs[(size_t)bi * kScatterSegs + seg] = ScatterSeg{region + first, c};
+        else
+            direct = 1;
+    }
+    s.bcnt[tid] = first;
+    s.direct[tid] = direct;
+    __syncthreads();
+
+    for (int e = tid; e < kScatterSet; e += kScatterThreads) {
+        const Key k = s.key[e];
+        if (k == kEmpty) continue;
+        const uint32_t f = (uint32_t)k;
+        const uint32_t vox = f & (kBlockVoxels - 1), sl = (f >> 9) & 255u, rank = f >> 17;
+        if (s.direct[sl])
+            scatter_direct(v, (size_t)s.bkey[sl] * kBlockVoxels + vox, (float)s.w[e], s.t[e]);
+        else
+            sc.entries[region + s.bcnt[sl] + rank] = make_uint2(vox | (s.w[e] << 9), __float_as_uint(s.t[e]));
+    }
+}
+
+// One touched block per workgroup at a time: its segments summed per voxel
+// in LDS, then each visited voxel read, capped and written once.
+__global__ __launch_bounds__(kScatterThreads) void k_scatter_apply(VolDev v, ScatterScratch sc) {
+    __shared__ uint32_t s_w[kBlockVoxels];
+    __shared__ float s_t[kBlockVoxels];
+    __shared__ int s_first[kScatterLists + 1];  // exclusive prefix of the list lengths
+    static_assert(kScatterLists == 64, "one lane per list");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (wave == 0) {
+        const int c = sc.touched_count[(sc.parity * kScatterLists + lane) * kScatterCtrStride];
+        int incl = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        s_first[lane + 1] = incl;
+        if (lane == 0) s_first[0] = 0;
+    }
+    __syncthreads();
+    const int n = s_first[kScatterLists];
     const bool capped = v.weight_cap > 0.0f;
-    for (int e = threadIdx.x; e < kScatterSet; e += blockDim.x) {
-        const unsigned long long k = s.key[e];
-        if (k == ~0ull) continue;
-        if (!capped)
-            scatter_global(v, (size_t)k, s.w[e], s.t[e]);
-        else if (s.w[e] > 0.0f)  // a slot can hold a voxel none of whose visits fit
-            scatter_global_capped(v, (size_t)k, s.w[e], s.t[e]);
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        int j = 0;  // the list holding item k: s_first[j] <= k < s_first[j + 1]
+        for (int half = kScatterLists / 2; half > 0; half >>= 1)
+            if (s_first[j + half] <= k) j += half;
+        const int32_t bi = sc.touched[(size_t)j * sc.list_cap + (k - s_first[j])];
+        for (int l = tid; l < kBlockVoxels; l += kScatterThreads) {
+            s_w[l] = 0u;
+            s_t[l] = 0.0f;
+        }
+        const int nseg = min(sc.seg_count[bi], kScatterSegs);
+        __syncthreads();
+        for (int g = wave; g < nseg; g += kScatterThreads / 64) {
+            const ScatterSeg d = sc.segs[(size_t)bi * kScatterSegs + g];
+            for (uint32_t e = lane; e < d.cnt; e += 64) {
+                const uint2 en = sc.entries[d.off + e];
+                atomicAdd(&s_w[en.x & (kBlockVoxels - 1)], en.x >> 9);
+                atomicAdd(&s_t[en.x & (kBlockVoxels - 1)], __uint_as_float(en.y));
+            }
+        }
+        __syncthreads();
+        for (int l = tid; l < kBlockVoxels; l += kScatterThreads) {
+            const uint32_t W = s_w[l];
+            if (W == 0u) continue;
+            const size_t idx = (size_t)bi * kBlockVoxels + l;
+            const float w = (float)W, T = s_t[l];
+            const float cur = v.weight[idx];
+            const float take = capped ? cap_take(cur, w, v.weight_cap) : w;
+            if (take <= 0.0f) continue;
+            v.weight[idx] = cur + take;
+            v.tsdf[idx] += take == w ? T : T * (take / w);
+            const float cc = 128.0f * take;
+            v.r[idx] += cc;
+            v.g[idx] += cc;
+            v.b[idx] += cc;
+        }
+        if (tid == 0) sc.seg_count[bi] = 0;
+        __syncthreads();
     }
 }
 
@@ -1767,8 +1899,20 @@ hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, hipStream_t s) {
-    k_scatter_update<<<(src.n + 255) / 256, 256, 0, s>>>(v, src);
+hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, const ScatterScratch& sc,
+                                 hipStream_t s) {
+    // the region of workgroup g starts at entry g * sc.region (32-bit)
+    const int64_t grid = ((int64_t)src.n + kScatterThreads - 1) / kScatterThreads;
+    if (src.n <= 0 || sc.region <= 0 || sc.region > kScatterSet ||
+        grid * sc.region > (int64_t)UINT32_MAX)
+        return hipErrorInvalidValue;
+    // voxel indices as 32-bit LDS keys while they fit (~0 stays free)
+    if ((uint64_t)v.pool_capacity * kBlockVoxels < 0xFFFFFFFFull)
+        k_scatter_visit<uint32_t><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    else
+        k_scatter_visit<unsigned long long><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    const int ag = v.pool_capacity < kScatterApplyGrid ? v.pool_capacity : kScatterApplyGrid;
+    k_scatter_apply<<<ag, kScatterThreads, 0, s>>>(v, sc);
     return hipGetLastError();
 }
 

@@ -30,6 +30,9 @@ namespace {
 
 constexpr uint32_t kScratchSlots = 1u << 20;  // reference types.hpp:30-33
 constexpr int kStages = 4;
+// scatter-update entries of one pass pair at most (8 B each): calls with more
+// points run as several pass pairs
+constexpr int64_t kScatterEntriesMax = int64_t(1) << 24;
 
 #define HIP_OK(expr)                                                                        \
     do {                                                                                    \
@@ -88,6 +91,14 @@ struct OsnTsdfHipVolume {
     // the depth image in the update's 8x4-tiled layout, grown on demand
     float* d_tiled = nullptr;
     size_t tiled_cap = 0;
+    // scatter update scratch: per-block parts at the first call, entries
+    // grown to the largest chunk of points
+    ScatterScratch sc{};
+    int64_t sc_entries_cap = 0;  // entries allocated
+    int32_t scatter_chunk() const {  // points per pass pair
+        const int64_t groups = kScatterEntriesMax / scatter_region(d.steps);
+        return (int32_t)(groups * kScatterThreads);
+    }
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -105,6 +116,7 @@ struct OsnTsdfHipVolume {
     bool grow_extract(int32_t blocks);
     bool grow_stage(int64_t verts, bool rgb);
     bool grow_tiled(int32_t W, int32_t H);
+    bool grow_scatter(int32_t n_points);
     // Wait for every stream that writes this volume: its own and, while it
     // belongs to a batch, the batch's (a non-blocking stream of its own).
     hipError_t sync() const {
@@ -172,6 +184,12 @@ bool OsnTsdfHipVolume::reset() {
     } else {
         HIP_OK(hipMemsetAsync(d.vox, 0, n_vox * voxel_bytes(d.storage), stream));
     }
+    if (sc.seg_count) {  // the scatter scratch's zero state, should a call have failed
+        HIP_OK(hipMemsetAsync(sc.seg_count, 0, (size_t)cfg.pool_capacity_blocks * sizeof(int32_t),
+                              stream));
+        HIP_OK(hipMemsetAsync(sc.touched_count, 0, kScatterCtrInts * sizeof(int32_t), stream));
+        sc.parity = 0;
+    }
     HIP_OK(hipStreamSynchronize(stream));
     calls = 0;
     ok = true;
@@ -187,7 +205,8 @@ OsnTsdfHipVolume::~OsnTsdfHipVolume() {
     for (auto e : ev_pool) hipEventDestroy(e);
     void* ptrs[] = {d.table, d.ctr, d.block_coord, d.tsdf, d.weight, d.r, d.g, d.b, d.vox,
                     d.new_keys, d.touched, d.sort_a, d.sort_b, d.assign_done, ex_vcount, ex_voff,
-                    ex_chunks, ex_nb, ex_flag_count, stage_posnor, stage_rgb, d_tiled};
+                    ex_chunks, ex_nb, ex_flag_count, stage_posnor, stage_rgb, d_tiled,
+                    sc.entries, sc.segs, sc.seg_count, sc.touched, sc.touched_count};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (stream) hipStreamDestroy(stream);
@@ -267,6 +286,31 @@ bool OsnTsdfHipVolume::grow_tiled(int32_t W, int32_t H) {
     tiled_cap = 0;
     HIP_OK(hipMalloc(&d_tiled, need * sizeof(float)));
     tiled_cap = need;
+    return true;
+}
+
+bool OsnTsdfHipVolume::grow_scatter(int32_t n_points) {
+    if (!sc.seg_count) {
+        const size_t pool = (size_t)cfg.pool_capacity_blocks;
+        HIP_OK(hipMalloc(&sc.segs, pool * kScatterSegs * sizeof(ScatterSeg)));
+        HIP_OK(hipMalloc(&sc.seg_count, pool * sizeof(int32_t)));
+        sc.list_cap = (int32_t)((pool + kScatterLists - 1) / kScatterLists);
+        HIP_OK(hipMalloc(&sc.touched, (size_t)sc.list_cap * kScatterLists * sizeof(int32_t)));
+        HIP_OK(hipMalloc(&sc.touched_count, kScatterCtrInts * sizeof(int32_t)));
+        HIP_OK(hipMemsetAsync(sc.seg_count, 0, pool * sizeof(int32_t), stream));
+        HIP_OK(hipMemsetAsync(sc.touched_count, 0, kScatterCtrInts * sizeof(int32_t), stream));
+        sc.parity = 0;
+        sc.region = scatter_region(d.steps);
+    }
+    const int64_t want = ((int64_t)n_points + kScatterThreads - 1) / kScatterThreads * sc.region;
+    if (want <= sc_entries_cap) return true;
+    // the stream may still read the old buffer
+    HIP_OK(hipStreamSynchronize(stream));
+    if (sc.entries) hipFree(sc.entries);
+    sc.entries = nullptr;
+    sc_entries_cap = 0;
+    HIP_OK(hipMalloc(&sc.entries, (size_t)want * sizeof(uint2)));
+    sc_entries_cap = want;
     return true;
 }
 
@@ -508,10 +552,21 @@ int32_t osn_tsdf_hip_update_voxels(OsnTsdfHipVolume* v, const float* d_positions
     if (!v || !d_positions || n_points <= 0) return 1;
     if (!v->ok) return 2;
     if (v->d.storage != kStoreSums) return 3;
+    // Chunks of points, each a call of the two-pass update: visits add up
+    // and the cap takes against the weight each chunk finds, so the sums and
+    // counts are those of one call over all points.
+    const int32_t per = v->scatter_chunk();
+    if (!v->grow_scatter(n_points < per ? n_points : per)) { v->ok = false; return 2; }
     hipEvent_t e;
     v->stage_begin(2, e);
-    const hipError_t err = launch_scatter_update(
-        v->d, points_src(d_positions, n_points, cam_x, cam_y, cam_z, radius_m), v->stream);
+    hipError_t err = hipSuccess;
+    for (int64_t at = 0; at < n_points && err == hipSuccess; at += per) {
+        const int32_t m = (int32_t)(n_points - at < per ? n_points - at : per);
+        err = launch_scatter_update(
+            v->d, points_src(d_positions + (size_t)at * 3, m, cam_x, cam_y, cam_z, radius_m),
+            v->sc, v->stream);
+        v->sc.parity ^= 1;
+    }
     v->stage_end(2, e);
     if (err != hipSuccess) { v->fail("update launch", err); return 2; }
     return 0;

@@ -146,7 +146,53 @@ hipError_t launch_tile_depth(const float* depth, int32_t W, int32_t H, float* ti
 hipError_t launch_assign(const VolDev& v, CallInfo call, hipStream_t s);
 hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStream_t s,
                                const LaunchEvents& ev = {});
-hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, hipStream_t s);
+// Scatter update (reference semantics) in two passes. The visit pass sums
+// each workgroup's visits per voxel in LDS and writes the distinct voxels as
+// entries grouped by pool block into the workgroup's own region of `entries`;
+// each (workgroup, block) group becomes one segment of that block. The apply
+// pass takes every touched block in one workgroup, sums its segments in LDS
+// and adds the totals to the voxel sums with plain loads and stores, the
+// weight cap applied once per voxel. Visits past a full LDS set, a full
+// per-workgroup block set or a block's kScatterSegs segments are added by
+// the visit pass itself with global atomics, before the apply pass runs.
+constexpr int kScatterThreads = 256;
+constexpr int kScatterSet = 3072;    // LDS voxel slots per workgroup (4 workgroups per CU)
+constexpr int kScatterBlocks = 256;  // distinct blocks per workgroup (one per thread)
+constexpr int kScatterSegs = 128;    // segments kept per block and call
+constexpr int kScatterApplyGrid = 2048;
+// Touched blocks are listed in kScatterLists lists by pool index mod
+// kScatterLists, so that first touches spread their atomics over as many
+// counters (kScatterCtrStride ints apart: one counter per 128 B).
+constexpr int kScatterLists = 64;
+constexpr int kScatterCtrStride = 32;
+constexpr int kScatterCtrInts = 2 * kScatterLists * kScatterCtrStride;
+
+struct ScatterSeg {
+    uint32_t off;  // first entry in `entries`
+    uint32_t cnt;
+};
+
+// A workgroup's distinct voxels number at most min(kScatterSet, its points x
+// samples per point): the stride of the regions in `entries`.
+__host__ __device__ inline int32_t scatter_region(int32_t steps) {
+    const int64_t visits = (int64_t)kScatterThreads * (2 * (int64_t)steps + 1);
+    return visits < kScatterSet ? (int32_t)visits : kScatterSet;
+}
+
+struct ScatterScratch {
+    uint2* entries;          // x = voxel in block | visits << 9, y = tsdf sum (float bits)
+    int32_t region;          // entries per workgroup region (scatter_region)
+    ScatterSeg* segs;        // kScatterSegs per pool block
+    int32_t* seg_count;      // per pool block, zero between calls
+    int32_t* touched;        // pool blocks with segments this call: list j at j * list_cap
+    int32_t* touched_count;  // per parity and list (kScatterCtrStride apart); the visit
+                             // pass zeroes the other parity's
+    int32_t list_cap;        // ceil(pool capacity / kScatterLists)
+    int32_t parity;
+};
+
+hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, const ScatterScratch& sc,
+                                 hipStream_t s);
 hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call, int32_t grid,
                               int32_t block_threads, hipStream_t s);
 // max_w_max: the largest weight cap of the batch's volumes (FP16 records
