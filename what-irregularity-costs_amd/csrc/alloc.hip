@@ -1587,6 +1587,36 @@ __device__ inline void scatter_direct(const VolDev& v, size_t idx, float w, floa
     else scatter_global(v, idx, w, t);
 }
 
+// Whether the lane to the left in the same 16-lane row holds the same key (a
+// row's first lane: never). Callers give lanes without a key distinct values.
+__device__ inline bool same_as_left(uint32_t me) {
+    const uint32_t left =
+        (uint32_t)__builtin_amdgcn_update_dpp((int)~me, (int)me, 0x111, 0xF, 0xF, false);  // row_shr:1
+    return left == me;
+}
+__device__ inline bool same_as_left(unsigned long long me) {
+    // both halves' moves run in every lane (no short-circuit)
+    const bool lo = same_as_left((uint32_t)me), hi = same_as_left((uint32_t)(me >> 32));
+    return lo && hi;
+}
+
+// One step of a suffix sum within runs: lane L adds the partial sums of lane
+// L + K when that lane is in the same 16-lane row and the same run. After
+// steps 1, 2, 4, 8 a run's first lane holds the whole run's visits and sum
+// (runs are at most a row long). Every lane must be active.
+template <int K>
+__device__ inline void run_step(int run, int& w, float& ts) {
+    constexpr int kRowShl = 0x100 + K;
+    const int r_o = __builtin_amdgcn_update_dpp(-1, run, kRowShl, 0xF, 0xF, false);
+    const int w_o = __builtin_amdgcn_update_dpp(0, w, kRowShl, 0xF, 0xF, false);
+    const float t_o =
+        __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(ts), kRowShl, 0xF, 0xF, false));
+    if (r_o == run) {
+        w += w_o;
+        ts += t_o;
+    }
+}
+
 template <typename Key>
 __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, PointSrc src,
                                                                    ScatterScratch sc) {
@@ -1619,9 +1649,13 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
         dist = sqrtf(d2);
         ok = ok && (dist > 1e-6f);
     }
-    if (ok) {
-        const float ux = dx / dist, uy = dy / dist, uz = dz / dist;
+    {
+        // Every lane runs every sample (the runs below compare neighbouring
+        // lanes); `val` says whether the lane's sample visits a voxel.
+        const float ux = ok ? dx / dist : 0.0f, uy = ok ? dy / dist : 0.0f,
+                    uz = ok ? dz / dist : 0.0f;
         const float inv_trunc = 1.0f / v.trunc;
+        const int lane = tid & 63;
         int64_t last_key = kEmptyKey;
         int32_t last_bi = kIdxAbsent;
         for (int32_t st = -v.steps; st <= v.steps; ++st) {
@@ -1634,14 +1668,31 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
             const float cz = ((float)vz + 0.5f) * v.voxel;
             const float ex = cx - src.cam[0], ey = cy - src.cam[1], ez = cz - src.cam[2];
             const float sdf = dist - sqrtf(ex * ex + ey * ey + ez * ez);
-            if (sdf < -v.trunc) continue;
+            bool val = ok && !(sdf < -v.trunc);
             const int32_t bx = floor_div8(vx), by = floor_div8(vy), bz = floor_div8(vz);
             const int64_t key = pack_coord(bx, by, bz);
-            if (key != last_key) { last_key = key; last_bi = find_index(v, key); }
-            if (last_bi < 0 || last_bi >= v.pool_capacity) continue;
+            if (val && key != last_key) { last_key = key; last_bi = find_index(v, key); }
+            val = val && last_bi >= 0 && last_bi < v.pool_capacity;
             const float sdf_n = fminf(fmaxf(sdf * inv_trunc, -1.0f), 1.0f);
-            const Key idx = (Key)last_bi * kBlockVoxels +
+            const Key idx = (Key)(val ? last_bi : 0) * kBlockVoxels +
                             (Key)(((vz & 7) * 8 + (vy & 7)) * 8 + (vx & 7));
+            // Neighbouring points of a wave often visit the same voxel at the
+            // same sample: runs of equal voxels in consecutive lanes (within
+            // a 16-lane row) are summed across lanes first, and only a run's
+            // first lane meets the LDS set.
+            const bool head = !same_as_left(val ? idx : ~Key(0) - (Key)lane);
+            const unsigned long long heads = __ballot(head);
+            // run number: the heads at or before this lane
+            const int run = __builtin_amdgcn_mbcnt_hi((uint32_t)(heads >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)heads, 0u)) +
+                            (head ? 1 : 0);
+            int w = val ? 1 : 0;
+            float ts = val ? sdf_n : 0.0f;
+            run_step<1>(run, w, ts);
+            run_step<2>(run, w, ts);
+            run_step<4>(run, w, ts);
+            run_step<8>(run, w, ts);
+            if (!(val && head)) continue;
             // multiply-shift onto [0, kScatterSet), which need not be a power of two
             uint32_t h = uint32_t((uint64_t(uint32_t((uint64_t(idx) * 0x9E3779B97F4A7C15ull) >> 32)) *
                                    kScatterSet) >> 32);
@@ -1649,14 +1700,14 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
             for (int p = 0; p < 16; ++p) {
                 const Key prev = atomicCAS(&s.key[h], kEmpty, idx);
                 if (prev == kEmpty || prev == idx) {
-                    atomicAdd(&s.w[h], 1u);
-                    atomicAdd(&s.t[h], sdf_n);
+                    atomicAdd(&s.w[h], (uint32_t)w);
+                    atomicAdd(&s.t[h], ts);
                     placed = true;
                     break;
                 }
                 h = h + 1 == kScatterSet ? 0u : h + 1;
             }
-            if (!placed) scatter_direct(v, (size_t)idx, 1.0f, sdf_n);  // LDS set crowded
+            if (!placed) scatter_direct(v, (size_t)idx, (float)w, ts);  // LDS set crowded
         }
     }
     __syncthreads();
@@ -1734,10 +1785,15 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
 // One touched block per workgroup at a time: its segments summed per voxel
 // in LDS, then each visited voxel read, capped and written once.
 __global__ __launch_bounds__(kScatterThreads) void k_scatter_apply(VolDev v, ScatterScratch sc) {
+    static_assert(kScatterLists == 64, "one lane per list");
+    static_assert(kScatterSegs == 2 * 64 && kBlockVoxels == 2 * kScatterThreads,
+                  "two waves hold a block's segments, two voxels per thread");
     __shared__ uint32_t s_w[kBlockVoxels];
     __shared__ float s_t[kBlockVoxels];
-    __shared__ int s_first[kScatterLists + 1];  // exclusive prefix of the list lengths
-    static_assert(kScatterLists == 64, "one lane per list");
+    __shared__ int s_first[kScatterLists + 1];       // exclusive prefix of the list lengths
+    __shared__ uint32_t s_soff[kScatterSegs];        // a segment's first entry
+    __shared__ uint32_t s_spre[kScatterSegs + 1];    // exclusive prefix of the segment lengths
+    __shared__ uint32_t s_wave0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (wave == 0) {
         const int c = sc.touched_count[(sc.parity * kScatterLists + lane) * kScatterCtrStride];
@@ -1757,35 +1813,66 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_apply(VolDev v, Sca
         for (int half = kScatterLists / 2; half > 0; half >>= 1)
             if (s_first[j + half] <= k) j += half;
         const int32_t bi = sc.touched[(size_t)j * sc.list_cap + (k - s_first[j])];
-        for (int l = tid; l < kBlockVoxels; l += kScatterThreads) {
-            s_w[l] = 0u;
-            s_t[l] = 0.0f;
+        const size_t base = (size_t)bi * kBlockVoxels;
+        // the thread's two voxels' sums, in flight while the segments are summed
+        float cw[2], ct[2], cr[2], cg[2], cb[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const size_t idx = base + tid + q * kScatterThreads;
+            cw[q] = v.weight[idx];
+            ct[q] = v.tsdf[idx];
+            cr[q] = v.r[idx];
+            cg[q] = v.g[idx];
+            cb[q] = v.b[idx];
+            s_w[tid + q * kScatterThreads] = 0u;
+            s_t[tid + q * kScatterThreads] = 0.0f;
         }
+        // segments: first entries and an exclusive prefix of their lengths
+        // (every slot's descriptor read beside the count, those past it unused)
+        ScatterSeg d{0u, 0u};
+        if (tid < kScatterSegs) d = sc.segs[(size_t)bi * kScatterSegs + tid];
         const int nseg = min(sc.seg_count[bi], kScatterSegs);
+        uint32_t c = 0;
+        if (tid < nseg) {
+            s_soff[tid] = d.off;
+            c = d.cnt;
+        }
+        uint32_t incl = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (tid == 63) s_wave0 = incl;
         __syncthreads();
-        for (int g = wave; g < nseg; g += kScatterThreads / 64) {
-            const ScatterSeg d = sc.segs[(size_t)bi * kScatterSegs + g];
-            for (uint32_t e = lane; e < d.cnt; e += 64) {
-                const uint2 en = sc.entries[d.off + e];
-                atomicAdd(&s_w[en.x & (kBlockVoxels - 1)], en.x >> 9);
-                atomicAdd(&s_t[en.x & (kBlockVoxels - 1)], __uint_as_float(en.y));
-            }
+        if (tid < kScatterSegs) s_spre[tid + 1] = incl + (wave == 1 ? s_wave0 : 0u);
+        if (tid == 0) s_spre[0] = 0u;
+        __syncthreads();
+        // every entry of every segment, all loads independent
+        const uint32_t total = s_spre[nseg];
+        for (uint32_t e = tid; e < total; e += kScatterThreads) {
+            int g = 0;  // the segment holding entry e: s_spre[g] <= e < s_spre[g + 1]
+            for (int half = kScatterSegs / 2; half > 0; half >>= 1)
+                if (g + half < nseg && s_spre[g + half] <= e) g += half;
+            const uint2 en = sc.entries[s_soff[g] + (e - s_spre[g])];
+            atomicAdd(&s_w[en.x & (kBlockVoxels - 1)], en.x >> 9);
+            atomicAdd(&s_t[en.x & (kBlockVoxels - 1)], __uint_as_float(en.y));
         }
         __syncthreads();
-        for (int l = tid; l < kBlockVoxels; l += kScatterThreads) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int l = tid + q * kScatterThreads;
             const uint32_t W = s_w[l];
             if (W == 0u) continue;
-            const size_t idx = (size_t)bi * kBlockVoxels + l;
+            const size_t idx = base + l;
             const float w = (float)W, T = s_t[l];
-            const float cur = v.weight[idx];
-            const float take = capped ? cap_take(cur, w, v.weight_cap) : w;
+            const float take = capped ? cap_take(cw[q], w, v.weight_cap) : w;
             if (take <= 0.0f) continue;
-            v.weight[idx] = cur + take;
-            v.tsdf[idx] += take == w ? T : T * (take / w);
+            v.weight[idx] = cw[q] + take;
+            v.tsdf[idx] = ct[q] + (take == w ? T : T * (take / w));
             const float cc = 128.0f * take;
-            v.r[idx] += cc;
-            v.g[idx] += cc;
-            v.b[idx] += cc;
+            v.r[idx] = cr[q] + cc;
+            v.g[idx] = cg[q] + cc;
+            v.b[idx] = cb[q] + cc;
         }
         if (tid == 0) sc.seg_count[bi] = 0;
         __syncthreads();
