@@ -133,6 +133,32 @@ def test_scatter_dense_clouds_exact(H, a, b, cap):
     assert n_vis > 8 * n_pts
 
 
+# A wider band: trunc 8 voxels gives 17 samples a point, 4,352 visits per
+# workgroup against the visit pass's 3,072-slot LDS set, so crowded sets add
+# the rest directly; weights still exact against the oracle.
+@pytest.mark.parametrize("cap", [0.0, 8.0])
+def test_scatter_wide_band_exact(H, cap):
+    pts = B.sphere_points(0.5, 400, 800)
+    gv = H.HipVolume(0.01, 1 << 15, 0.08, cap)
+    assert gv.integrate(dev(pts), (0, 0, 0)) == 0
+    ov = B.Volume(0.01, 1 << 15, 0.08, cap)
+    ov.integrate(pts, (0, 0, 0))
+    ou = B.Volume(0.01, 1 << 15, 0.08, 0.0) if cap else ov
+    if cap:
+        ou.integrate(pts, (0, 0, 0))
+    gk = gv.keys()
+    assert np.array_equal(np.sort(gk), np.sort(ov.keys()))
+    t, w = gv.read_voxels()
+    for i, k in enumerate(gk):
+        ow = ov.block_voxels(int(k))[1]
+        ut, uw = ou.block_voxels(int(k))[:2]
+        assert np.array_equal(w[i], ow), f"weights differ in block {k}"
+        free = uw <= (np.ceil(cap) if cap else np.inf)
+        err = np.abs(t[i][free].astype(np.float64) - ut[free])
+        assert np.all(err <= sum_tol(uw[free])), f"tsdf sums differ in block {k}"
+    print(f"{len(gk)} blocks, trunc 8 voxels, cap {cap}")
+
+
 def test_sphere_analytic_gate_and_known_answer(H):
     """cpp/tests/test_cuda_analytic.cu:185-227 with the default cap of 32;
     1,160 blocks and 846,288 vertices are the published counts

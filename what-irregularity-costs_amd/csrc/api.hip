@@ -13,10 +13,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/osn_tsdf_hip.h"
@@ -61,6 +63,88 @@ size_t voxel_bytes(int32_t storage) {
     }
 }
 
+// Copies into a caller's pageable host memory, split over a few threads: the
+// reference ABI's extract_mesh fills such a buffer, and one thread's memcpy
+// (what a pageable device-to-host copy ends in) runs well below what the
+// link and the host memory allow. One job at a time; the calling thread takes
+// a share too.
+class CopyPool {
+public:
+    static CopyPool& get() {
+        static CopyPool pool;
+        return pool;
+    }
+
+    void copy(void* dst, const void* src, size_t n) {
+        if (n < (size_t(1) << 20) || workers_.empty()) {
+            std::memcpy(dst, src, n);
+            return;
+        }
+        std::lock_guard<std::mutex> job(job_m_);
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            dst_ = static_cast<char*>(dst);
+            src_ = static_cast<const char*>(src);
+            n_ = n;
+            pending_ = (int)workers_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        part(0);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    CopyPool() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int n = (int)std::min(7u, hw > 1 ? hw - 1 : 0u);
+        for (int i = 0; i < n; ++i) workers_.emplace_back([this, i] { work(i + 1); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    // part p of parts() equal pieces, 64-byte aligned
+    void part(int p) {
+        const size_t parts = workers_.size() + 1;
+        const size_t step = ((n_ + parts - 1) / parts + 63) & ~size_t(63);
+        const size_t lo = std::min(n_, step * p), hi = std::min(n_, lo + step);
+        if (hi > lo) std::memcpy(dst_ + lo, src_ + lo, hi - lo);
+    }
+    void work(int id) {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            part(id);
+            std::lock_guard<std::mutex> lk(m_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+
+    std::vector<std::thread> workers_;
+    std::mutex job_m_, m_;
+    std::condition_variable cv_, done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t n_ = 0;
+    uint64_t gen_ = 0;
+    int pending_ = 0;
+    bool stop_ = false;
+};
+
+// Pinned host staging for device-to-pageable copies: two halves of this size.
+constexpr size_t kPinnedChunk = size_t(4) << 20;
+
 }  // namespace
 
 struct OsnTsdfHipVolume {
@@ -87,6 +171,10 @@ struct OsnTsdfHipVolume {
     int32_t ex_blocks_cap = 0;
     float* stage_posnor = nullptr;
     uint8_t* stage_rgb = nullptr;
+    // pinned host staging (2 x kPinnedChunk) and its chunk events, at the first
+    // extract_mesh into a host buffer
+    char* pinned = nullptr;
+    hipEvent_t pinned_ev[2] = {nullptr, nullptr};
     int64_t stage_cap = 0;
     // the depth image in the update's 8x4-tiled layout, grown on demand
     float* d_tiled = nullptr;
@@ -116,6 +204,7 @@ struct OsnTsdfHipVolume {
     bool grow_extract(int32_t blocks);
     bool grow_stage(int64_t verts, bool rgb);
     bool grow_tiled(int32_t W, int32_t H);
+    bool copy_to_host(void* dst, const void* src, size_t n);
     bool grow_scatter(int32_t n_points);
     // Wait for every stream that writes this volume: its own and, while it
     // belongs to a batch, the batch's (a non-blocking stream of its own).
@@ -203,6 +292,9 @@ OsnTsdfHipVolume::~OsnTsdfHipVolume() {
         hipEventDestroy(p.second.second);
     }
     for (auto e : ev_pool) hipEventDestroy(e);
+    for (auto e : pinned_ev)
+        if (e) hipEventDestroy(e);
+    if (pinned) hipHostFree(pinned);
     void* ptrs[] = {d.table, d.ctr, d.block_coord, d.tsdf, d.weight, d.r, d.g, d.b, d.vox,
                     d.new_keys, d.touched, d.sort_a, d.sort_b, d.assign_done, ex_vcount, ex_voff,
                     ex_chunks, ex_nb, ex_flag_count, stage_posnor, stage_rgb, d_tiled,
@@ -286,6 +378,42 @@ bool OsnTsdfHipVolume::grow_tiled(int32_t W, int32_t H) {
     tiled_cap = 0;
     HIP_OK(hipMalloc(&d_tiled, need * sizeof(float)));
     tiled_cap = need;
+    return true;
+}
+
+// n bytes from device memory to a pageable host buffer: chunks go by DMA into
+// the two pinned halves in turn, and each chunk's copy into dst (on the copy
+// pool) runs while the next one's DMA does. Ends with the stream drained.
+bool OsnTsdfHipVolume::copy_to_host(void* dst, const void* src, size_t n) {
+    if (n == 0) return true;
+    if (!pinned) {
+        if (hipHostMalloc((void**)&pinned, 2 * kPinnedChunk, hipHostMallocDefault) != hipSuccess) {
+            pinned = nullptr;  // no pinned memory: one pageable copy
+            HIP_OK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            return true;
+        }
+        for (auto& e : pinned_ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    CopyPool& pool = CopyPool::get();
+    char* out = static_cast<char*>(dst);
+    const char* in = static_cast<const char*>(src);
+    size_t prev_off = 0, prev_len = 0;
+    int k = 0;
+    for (size_t off = 0; off < n; off += kPinnedChunk, ++k) {
+        const size_t len = std::min(kPinnedChunk, n - off);
+        HIP_OK(hipMemcpyAsync(pinned + (k & 1) * kPinnedChunk, in + off, len,
+                              hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipEventRecord(pinned_ev[k & 1], stream));
+        if (k > 0) {
+            HIP_OK(hipEventSynchronize(pinned_ev[(k - 1) & 1]));
+            pool.copy(out + prev_off, pinned + ((k - 1) & 1) * kPinnedChunk, prev_len);
+        }
+        prev_off = off;
+        prev_len = len;
+    }
+    HIP_OK(hipEventSynchronize(pinned_ev[(k - 1) & 1]));
+    pool.copy(out + prev_off, pinned + ((k - 1) & 1) * kPinnedChunk, prev_len);
     return true;
 }
 
@@ -845,13 +973,9 @@ int32_t osn_tsdf_hip_extract_mesh(OsnTsdfHipVolume* v, float* buffer_posnor, uin
     err = launch_extract(v->d, a, true, v->stream);
     v->stage_end(3, e);
     if (err != hipSuccess) { v->fail("extract emit launch", err); return -1; }
-    if (hipMemcpyAsync(buffer_posnor, v->stage_posnor, (size_t)total * 6 * sizeof(float),
-                       hipMemcpyDeviceToHost, v->stream) != hipSuccess)
+    if (!v->copy_to_host(buffer_posnor, v->stage_posnor, (size_t)total * 6 * sizeof(float)))
         return -1;
-    if (buffer_rgb && hipMemcpyAsync(buffer_rgb, v->stage_rgb, (size_t)total * 3,
-                                     hipMemcpyDeviceToHost, v->stream) != hipSuccess)
-        return -1;
-    if (hipStreamSynchronize(v->stream) != hipSuccess) return -1;
+    if (buffer_rgb && !v->copy_to_host(buffer_rgb, v->stage_rgb, (size_t)total * 3)) return -1;
     return total;
 }
 
