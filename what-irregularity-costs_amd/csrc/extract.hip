@@ -45,6 +45,7 @@ constexpr int emit_waves_per_simd() { return STORE == kStoreSums ? 2 : (STORE ==
 constexpr int kCube = 9 * 9 * 9;
 constexpr int kScanGrid = 256;      // scan workgroups (grid-stride over chunks)
 constexpr int kSerialChunks = 64;   // up to this many chunks one workgroup scans alone
+constexpr int kStripeChunks = 16;   // chunks it loads and scans at a time
 constexpr uint32_t kScanGridBatch = 32;  // the same per volume of a batch
 constexpr int kSlotCap = 768;         // triangles listed per pass over a block
 constexpr int kStageVerts = 3 * kWaveThreads;  // one group: a triangle per lane
@@ -709,38 +710,50 @@ __device__ void scan_chunks(const VolDev& v, const ExtractArgs& a) {
         return;
     }
     if (n_chunks <= kSerialChunks) {
-        // Small volumes: workgroup 0 alone, each thread a run of up to
-        // kSerialChunks consecutive blocks; offsets are absolute and the
-        // chunk sums zero. No fences, no arrival counter.
+        // Up to 64 chunks (65,536 blocks): workgroup 0 alone, 16 chunks at a
+        // time. Thread t holds the counts of blocks t + 1024 j of the group's
+        // chunks (one coalesced load per chunk, all in flight together); the
+        // chunks are scanned across the threads at once, and a block's offset
+        // is the earlier chunks' totals plus the scan within its chunk.
         if (blockIdx.x != 0) return;
-        const int first = threadIdx.x * n_chunks;
-        const int last = min(first + n_chunks, n);
-        // a thread's run is read 8 counts at a time, every load of a batch
-        // in flight before the first is used
-        constexpr int kBatch = 8;
-        int sum = 0;
-        for (int k0 = first; k0 < last; k0 += kBatch) {
-            int c[kBatch];
+        __shared__ int s_tot[kStripeChunks * (kScanChunk / 64)];
+        const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+        long long carry = 0;  // vertices of the chunks before the current one
+        for (int g0 = 0; g0 < n_chunks; g0 += kStripeChunks) {
+            int c[kStripeChunks], incl[kStripeChunks];
 #pragma unroll
-            for (int j = 0; j < kBatch; ++j) c[j] = k0 + j < last ? global(a.block_vcount)[k0 + j] : 0;
-#pragma unroll
-            for (int j = 0; j < kBatch; ++j) sum += c[j];
-        }
-        int total;  // <= 64 x 1024 blocks x 18432 vertices < 2^31
-        int run = wg_exclusive_scan(sum, s_wave, total);
-        for (int k0 = first; k0 < last; k0 += kBatch) {
-            int c[kBatch];
-#pragma unroll
-            for (int j = 0; j < kBatch; ++j) c[j] = k0 + j < last ? global(a.block_vcount)[k0 + j] : 0;
-#pragma unroll
-            for (int j = 0; j < kBatch; ++j) {
-                if (k0 + j < last) global(a.block_voff)[k0 + j] = run;
-                run += c[j];
+            for (int j = 0; j < kStripeChunks; ++j) {
+                const int b = (g0 + j) * kScanChunk + t;
+                c[j] = (g0 + j < n_chunks && b < n) ? global(a.block_vcount)[b] : 0;
             }
+#pragma unroll
+            for (int j = 0; j < kStripeChunks; ++j) {
+                incl[j] = c[j];
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int y = __shfl_up(incl[j], o);
+                    if (lane >= o) incl[j] += y;
+                }
+                if (lane == 63) s_tot[j * (kScanChunk / 64) + wave] = incl[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kStripeChunks; ++j) {
+                int before = 0, tot = 0;
+                for (int w = 0; w < kScanChunk / 64; ++w) {
+                    const int x = s_tot[j * (kScanChunk / 64) + w];
+                    if (w < wave) before += x;
+                    tot += x;
+                }
+                const int b = (g0 + j) * kScanChunk + t;
+                if (g0 + j < n_chunks && b < n)
+                    global(a.block_voff)[b] = (int)(carry + before + incl[j] - c[j]);
+                carry += tot;
+            }
+            __syncthreads();  // s_tot is rewritten by the next group
         }
-        if ((int)threadIdx.x < n_chunks) a.chunk_sums[threadIdx.x] = 0;
-        if (threadIdx.x == 0) {
-            const int32_t out = total > a.cap ? -1 : total;
+        if (t < n_chunks) a.chunk_sums[t] = 0;
+        if (t == 0) {
+            const int32_t out = carry > (long long)a.cap ? -1 : (int32_t)carry;
             *a.count_out = out;
             v.ctr->mesh_vertices = out;
         }
