@@ -99,7 +99,15 @@ private:
     CopyPool() {
         const unsigned hw = std::thread::hardware_concurrency();
         const int n = (int)std::min(7u, hw > 1 ? hw - 1 : 0u);
-        for (int i = 0; i < n; ++i) workers_.emplace_back([this, i] { work(i + 1); });
+        // a thread that cannot be started leaves the pool smaller (no
+        // exception may leave a C entry point)
+        for (int i = 0; i < n; ++i) {
+            try {
+                workers_.emplace_back([this, i] { work(i + 1); });
+            } catch (...) {
+                break;
+            }
+        }
     }
     ~CopyPool() {
         {
