@@ -75,13 +75,17 @@ int32_t osn_tsdf_hip_device_view(const OsnTsdfHipVolume* v, OsnTsdfDeviceView* o
 int32_t osn_tsdf_hip_allocate_blocks(OsnTsdfHipVolume* v, const float* d_positions,
                                      int32_t n_points, float cam_x, float cam_y, float cam_z,
                                      float radius_m);
-/* c_api.h:92-94 osn_tsdf_cuda_update_voxels (scatter sums, weighted-sum storage only) */
+/* c_api.h:92-94 osn_tsdf_cuda_update_voxels (scatter sums, weighted-sum storage only).
+ * The first call allocates the update's scratch (about 1 KiB per pool block, and
+ * entries for up to 1.86 M points at the default band); larger calls run in parts. */
 int32_t osn_tsdf_hip_update_voxels(OsnTsdfHipVolume* v, const float* d_positions,
                                    int32_t n_points, float cam_x, float cam_y, float cam_z,
                                    float radius_m);
 /* c_api.h:97-98 osn_tsdf_cuda_extract_mesh: HOST buffers, 6 floats per
  * vertex, rgb may be NULL. Vertex count, or -1 when vert_cap is too small
- * (nothing written) or on error. */
+ * (nothing written) or on error. The copy goes through 8 MiB of pinned host
+ * memory per volume (taken at the first call) and a process-wide pool of up
+ * to 7 copy threads. */
 int32_t osn_tsdf_hip_extract_mesh(OsnTsdfHipVolume* v, float* buffer_posnor, uint8_t* buffer_rgb,
                                   int32_t vert_cap, float min_weight, float iso);
 /* c_api.h:106 osn_tsdf_cuda_reset */
