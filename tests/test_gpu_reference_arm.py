@@ -97,7 +97,9 @@ def test_scatter_weight_cap_counts_exact(H, cap):
 # Dense clouds: the 1.28M-point sphere gives most blocks more than 64
 # workgroup segments in one call, so the visit pass adds part of their visits
 # directly; 2.2M points take two chunks of the two-pass update in one call.
-# Weights must still equal the sequential oracle's, capped or not. Near the
+# Weights must still equal the sequential oracle's, capped or not, after two
+# calls (the second takes the visit pass's smaller LDS set: the first call
+# saw many points per touched block). Near the
 # poles a voxel collects thousands of visits, where two FP32 sums of the same
 # n terms in different orders may differ by up to 2 (n - 1) 2^-24 sum|terms|
 # (sum|terms| <= n = the weight), so the tolerance grows to that bound.
@@ -110,12 +112,14 @@ def sum_tol(w):
 def test_scatter_dense_clouds_exact(H, a, b, cap):
     pts = B.sphere_points(0.5, a, b)
     gv = H.HipVolume(0.01, 1 << 14, -1.0, cap)
-    assert gv.integrate(dev(pts), (0, 0, 0)) == 0
     ov = B.Volume(0.01, 1 << 14, -1.0, cap)
-    ov.integrate(pts, (0, 0, 0))
     ou = B.Volume(0.01, 1 << 14, -1.0, 0.0) if cap else ov
-    if cap:
-        ou.integrate(pts, (0, 0, 0))
+    d = dev(pts)
+    for _ in range(2):
+        assert gv.integrate(d, (0, 0, 0)) == 0
+        ov.integrate(pts, (0, 0, 0))
+        if cap:
+            ou.integrate(pts, (0, 0, 0))
     gk = gv.keys()
     assert np.array_equal(np.sort(gk), np.sort(ov.keys()))
     t, w = gv.read_voxels()
@@ -129,8 +133,8 @@ def test_scatter_dense_clouds_exact(H, a, b, cap):
         assert np.all(err <= sum_tol(uw[free])), f"tsdf sums differ in block {k}"
         n_vis += int(uw.sum())
     n_pts = len(pts) // 3
-    print(f"{n_pts} points, {n_vis} visits, {len(gk)} blocks")
-    assert n_vis > 8 * n_pts
+    print(f"{n_pts} points x 2 calls, {n_vis} visits, {len(gk)} blocks")
+    assert n_vis > 16 * n_pts
 
 
 # A wider band: trunc 8 voxels gives 17 samples a point, 4,352 visits per

@@ -1522,16 +1522,16 @@ __global__ __launch_bounds__(1024) void k_assign_batch(BatchDev b) {
 
 static_assert(kScatterBlocks == kScatterThreads, "one block slot per thread");
 static_assert(kScatterBlocks == 256, "the block-slot hash takes the top 8 bits");
-static_assert(kBlockVoxels == 512 && kScatterSet <= 4096,
+static_assert(kBlockVoxels == 512 && kScatterSet <= 4096 && kScatterSetDense <= kScatterSet,
               "a grouped entry packs voxel (9 bits), block slot (8) and rank (12)");
 
-template <typename Key>
+template <typename Key, int SET>
 struct ScatterVisitLds {
     // voxel index in the pool, ~0 = empty; once grouped by block, the voxel
     // in its block | block slot << 9 | rank among the block's entries << 17
-    Key key[kScatterSet];
-    uint32_t w[kScatterSet];         // visits
-    float t[kScatterSet];            // sum of the visits' normalised distances
+    Key key[SET];
+    uint32_t w[SET];                 // visits
+    float t[SET];                    // sum of the visits' normalised distances
     uint32_t bkey[kScatterBlocks];   // pool block, ~0 = empty
     uint32_t bcnt[kScatterBlocks];   // the block's entries, then its first entry
     uint32_t wsum[kScatterThreads / 64];
@@ -1617,13 +1617,13 @@ __device__ inline void run_step(int run, int& w, float& ts) {
     }
 }
 
-template <typename Key>
+template <typename Key, int SET>
 __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, PointSrc src,
                                                                    ScatterScratch sc) {
-    __shared__ ScatterVisitLds<Key> s;
+    __shared__ ScatterVisitLds<Key, SET> s;
     constexpr Key kEmpty = ~Key(0);
     const int tid = threadIdx.x;
-    for (int e = tid; e < kScatterSet; e += kScatterThreads) {
+    for (int e = tid; e < SET; e += kScatterThreads) {
         s.key[e] = kEmpty;
         s.w[e] = 0u;
         s.t[e] = 0.0f;
@@ -1693,9 +1693,9 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
             run_step<4>(run, w, ts);
             run_step<8>(run, w, ts);
             if (!(val && head)) continue;
-            // multiply-shift onto [0, kScatterSet), which need not be a power of two
+            // multiply-shift onto [0, SET), which need not be a power of two
             uint32_t h = uint32_t((uint64_t(uint32_t((uint64_t(idx) * 0x9E3779B97F4A7C15ull) >> 32)) *
-                                   kScatterSet) >> 32);
+                                   SET) >> 32);
             bool placed = false;
             for (int p = 0; p < 16; ++p) {
                 const Key prev = atomicCAS(&s.key[h], kEmpty, idx);
@@ -1705,7 +1705,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
                     placed = true;
                     break;
                 }
-                h = h + 1 == kScatterSet ? 0u : h + 1;
+                h = h + 1 == SET ? 0u : h + 1;
             }
             if (!placed) scatter_direct(v, (size_t)idx, (float)w, ts);  // LDS set crowded
         }
@@ -1713,7 +1713,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
     __syncthreads();
 
     // the distinct voxels by block: a slot per block, a rank per entry
-    for (int e = tid; e < kScatterSet; e += kScatterThreads) {
+    for (int e = tid; e < SET; e += kScatterThreads) {
         const Key k = s.key[e];
         if (k == kEmpty) continue;
         const uint32_t bi = (uint32_t)(k / kBlockVoxels);
@@ -1770,7 +1770,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
     s.direct[tid] = direct;
     __syncthreads();
 
-    for (int e = tid; e < kScatterSet; e += kScatterThreads) {
+    for (int e = tid; e < SET; e += kScatterThreads) {
         const Key k = s.key[e];
         if (k == kEmpty) continue;
         const uint32_t f = (uint32_t)k;
@@ -1807,6 +1807,8 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_apply(VolDev v, Sca
     }
     __syncthreads();
     const int n = s_first[kScatterLists];
+    // the call's touched blocks, for the host's choice of set size next call
+    if (blockIdx.x == 0 && tid == 0 && sc.touched_out) *sc.touched_out = n;
     const bool capped = v.weight_cap > 0.0f;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         int j = 0;  // the list holding item k: s_first[j] <= k < s_first[j + 1]
@@ -1987,17 +1989,23 @@ hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStre
 }
 
 hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, const ScatterScratch& sc,
-                                 hipStream_t s) {
+                                 bool dense, hipStream_t s) {
     // the region of workgroup g starts at entry g * sc.region (32-bit)
     const int64_t grid = ((int64_t)src.n + kScatterThreads - 1) / kScatterThreads;
     if (src.n <= 0 || sc.region <= 0 || sc.region > kScatterSet ||
         grid * sc.region > (int64_t)UINT32_MAX)
         return hipErrorInvalidValue;
     // voxel indices as 32-bit LDS keys while they fit (~0 stays free)
-    if ((uint64_t)v.pool_capacity * kBlockVoxels < 0xFFFFFFFFull)
-        k_scatter_visit<uint32_t><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    const bool small_keys = (uint64_t)v.pool_capacity * kBlockVoxels < 0xFFFFFFFFull;
+    if (small_keys && dense)
+        k_scatter_visit<uint32_t, kScatterSetDense><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    else if (small_keys)
+        k_scatter_visit<uint32_t, kScatterSet><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    else if (dense)
+        k_scatter_visit<unsigned long long, kScatterSetDense>
+            <<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
     else
-        k_scatter_visit<unsigned long long><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+        k_scatter_visit<unsigned long long, kScatterSet><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
     const int ag = v.pool_capacity < kScatterApplyGrid ? v.pool_capacity : kScatterApplyGrid;
     k_scatter_apply<<<ag, kScatterThreads, 0, s>>>(v, sc);
     return hipGetLastError();

@@ -191,6 +191,10 @@ struct OsnTsdfHipVolume {
     // grown to the largest chunk of points
     ScatterScratch sc{};
     int64_t sc_entries_cap = 0;  // entries allocated
+    // pinned, device-mapped: the touched-block count of the last update pass pair
+    // (written by its apply pass, read here without waiting: a hint for the next
+    // call's LDS set size, whatever call it is from)
+    int32_t* sc_touched_host = nullptr;
     int32_t scatter_chunk() const {  // points per pass pair
         const int64_t groups = kScatterEntriesMax / scatter_region(d.steps);
         return (int32_t)(groups * kScatterThreads);
@@ -303,6 +307,7 @@ OsnTsdfHipVolume::~OsnTsdfHipVolume() {
     for (auto e : pinned_ev)
         if (e) hipEventDestroy(e);
     if (pinned) hipHostFree(pinned);
+    if (sc_touched_host) hipHostFree(sc_touched_host);
     void* ptrs[] = {d.table, d.ctr, d.block_coord, d.tsdf, d.weight, d.r, d.g, d.b, d.vox,
                     d.new_keys, d.touched, d.sort_a, d.sort_b, d.assign_done, ex_vcount, ex_voff,
                     ex_chunks, ex_nb, ex_flag_count, stage_posnor, stage_rgb, d_tiled,
@@ -437,6 +442,16 @@ bool OsnTsdfHipVolume::grow_scatter(int32_t n_points) {
         HIP_OK(hipMemsetAsync(sc.touched_count, 0, kScatterCtrInts * sizeof(int32_t), stream));
         sc.parity = 0;
         sc.region = scatter_region(d.steps);
+        // without the hint every call takes the larger set
+        sc.touched_out = nullptr;
+        if (hipHostMalloc((void**)&sc_touched_host, sizeof(int32_t), hipHostMallocMapped) ==
+            hipSuccess) {
+            *sc_touched_host = 0;
+            if (hipHostGetDevicePointer((void**)&sc.touched_out, sc_touched_host, 0) != hipSuccess)
+                sc.touched_out = nullptr;
+        } else {
+            sc_touched_host = nullptr;
+        }
     }
     const int64_t want = ((int64_t)n_points + kScatterThreads - 1) / kScatterThreads * sc.region;
     if (want <= sc_entries_cap) return true;
@@ -698,9 +713,12 @@ int32_t osn_tsdf_hip_update_voxels(OsnTsdfHipVolume* v, const float* d_positions
     hipError_t err = hipSuccess;
     for (int64_t at = 0; at < n_points && err == hipSuccess; at += per) {
         const int32_t m = (int32_t)(n_points - at < per ? n_points - at : per);
+        // dense when the last pass pair saw many points per touched block
+        const int32_t last = v->sc.touched_out ? __atomic_load_n(v->sc_touched_host, __ATOMIC_RELAXED) : 0;
+        const bool dense = last > 0 && (int64_t)m >= (int64_t)kScatterDensePoints * last;
         err = launch_scatter_update(
             v->d, points_src(d_positions + (size_t)at * 3, m, cam_x, cam_y, cam_z, radius_m),
-            v->sc, v->stream);
+            v->sc, dense, v->stream);
         v->sc.parity ^= 1;
     }
     v->stage_end(2, e);

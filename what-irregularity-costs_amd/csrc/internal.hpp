@@ -156,7 +156,14 @@ hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStre
 // per-workgroup block set or a block's kScatterSegs segments are added by
 // the visit pass itself with global atomics, before the apply pass runs.
 constexpr int kScatterThreads = 256;
-constexpr int kScatterSet = 3072;    // LDS voxel slots per workgroup (4 workgroups per CU)
+// LDS voxel slots per workgroup: 3,072 (4 workgroups per CU) for sparse
+// clouds, whose workgroups see up to ~1,600 distinct voxels; 2,048 (5 per CU)
+// for dense ones, where more workgroups per CU pay and the set stays below
+// half full.
+constexpr int kScatterSet = 3072;
+constexpr int kScatterSetDense = 2048;
+// Points per touched block at which a call counts as dense.
+constexpr int kScatterDensePoints = 100;
 constexpr int kScatterBlocks = 256;  // distinct blocks per workgroup (one per thread)
 constexpr int kScatterSegs = 128;    // segments kept per block and call
 constexpr int kScatterApplyGrid = 2048;
@@ -188,11 +195,14 @@ struct ScatterScratch {
     int32_t* touched_count;  // per parity and list (kScatterCtrStride apart); the visit
                              // pass zeroes the other parity's
     int32_t list_cap;        // ceil(pool capacity / kScatterLists)
+    int32_t* touched_out;    // device view of a host int: the apply pass leaves the call's
+                             // touched-block count there (may be null)
     int32_t parity;
 };
 
+// dense: the visit pass takes the smaller LDS set (kScatterSetDense).
 hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, const ScatterScratch& sc,
-                                 hipStream_t s);
+                                 bool dense, hipStream_t s);
 hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call, int32_t grid,
                               int32_t block_threads, hipStream_t s);
 // max_w_max: the largest weight cap of the batch's volumes (FP16 records
