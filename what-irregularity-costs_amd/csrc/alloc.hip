@@ -316,12 +316,29 @@ __device__ __attribute__((always_inline)) void table_keys(const TileCtx& c, bool
     // consecutive slots per round trip, so a wave no longer runs as many
     // rounds as its deepest lane has slots to inspect.
     bool stopped = false;  // a lane that reached an empty slot without inserting (no-CAS mode)
+    // The first call after a reset finds no key of an earlier call in the
+    // table, so a lane may try its compare-and-swap at once: the old key it
+    // returns says inserted (empty), found (a key another workgroup of this
+    // call inserted: its index is still pending) or occupied (probe on), one
+    // round trip a slot instead of a read and then the exchange. Calls that
+    // track touched blocks need the slot's index and stamp, and keep the read.
+    const bool cas_first = MODE == kAllocNormal && !TOUCH && c.stamp == 1;
     if (go) {
         slot = home_slot(v, key);
         const uint32_t own =
             COOP && v.probe_limit > kLaneProbes ? kLaneProbes : v.probe_limit;
         for (uint32_t p = 0; p < own; ++p) {
             ++steps;
+            if (cas_first) {
+                int64_t prev = kEmptyKey;
+                __hip_atomic_compare_exchange_strong(&glob(v.table)[slot].key, &prev, key,
+                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                if (prev == kEmptyKey) { inserted = true; break; }
+                if (prev == key) { found = true; break; }
+                slot = next_slot(v, slot);
+                continue;
+            }
             // key, index and stamp of the slot in one round trip: no index
             // changes during this phase, and a key inserted during it still
             // holds kIdxPending and stamp 0 from the empty slot
