@@ -163,6 +163,36 @@ def test_scatter_wide_band_exact(H, cap):
     print(f"{len(gk)} blocks, trunc 8 voxels, cap {cap}")
 
 
+# Calls of up to 2^17 points take one sample per lane; the sums and weights
+# are those of the band layout. Two calls, capped and not, a 4 cm and an 8 cm
+# band.
+@pytest.mark.parametrize("trunc,cap", [(-1.0, 0.0), (-1.0, 8.0), (0.08, 0.0)])
+def test_scatter_small_calls_exact(H, trunc, cap):
+    pts = B.sphere_points(0.5, 200, 400)  # 80,000 points
+    gv = H.HipVolume(0.01, 1 << 15, trunc, cap)
+    ov = B.Volume(0.01, 1 << 15, trunc, cap)
+    ou = B.Volume(0.01, 1 << 15, trunc, 0.0) if cap else ov
+    d = dev(pts)
+    for _ in range(2):
+        assert gv.integrate(d, (0, 0, 0)) == 0
+        ov.integrate(pts, (0, 0, 0))
+        if cap:
+            ou.integrate(pts, (0, 0, 0))
+    gk = gv.keys()
+    assert np.array_equal(np.sort(gk), np.sort(ov.keys()))
+    t, w = gv.read_voxels()
+    n_vis = 0
+    for i, k in enumerate(gk):
+        ow = ov.block_voxels(int(k))[1]
+        ut, uw = ou.block_voxels(int(k))[:2]
+        assert np.array_equal(w[i], ow), f"weights differ in block {k}"
+        free = uw <= (np.ceil(cap) if cap else np.inf)
+        err = np.abs(t[i][free].astype(np.float64) - ut[free])
+        assert np.all(err <= sum_tol(uw[free])), f"tsdf sums differ in block {k}"
+        n_vis += int(uw.sum())
+    assert n_vis > 2 * 8 * (len(pts) // 3)
+
+
 def test_sphere_analytic_gate_and_known_answer(H):
     """cpp/tests/test_cuda_analytic.cu:185-227 with the default cap of 32;
     1,160 blocks and 846,288 vertices are the published counts

@@ -1634,7 +1634,14 @@ __device__ inline void run_step(int run, int& w, float& ts) {
     }
 }
 
-template <typename Key, int SET>
+// SPLIT (small calls): a lane takes one sample of one point instead of the
+// point's whole band. Wave q of a point group's waves takes sample q of its
+// 64 consecutive points, so the sample's chain (ray, voxel, lookup, set
+// insert) is a lane's whole work and small calls fill more of the chip; the
+// runs of neighbouring lanes are the same sample of neighbouring points, as
+// in the band layout. A group's waves are dealt kScatterThreads / 64 to a
+// workgroup (the last workgroup's spare waves have no sample).
+template <typename Key, int SET, bool SPLIT>
 __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, PointSrc src,
                                                                    ScatterScratch sc) {
     __shared__ ScatterVisitLds<Key, SET> s;
@@ -1653,7 +1660,17 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
         sc.touched_count[((sc.parity ^ 1) * kScatterLists + tid) * kScatterCtrStride] = 0;
     __syncthreads();
 
-    const int i = blockIdx.x * kScatterThreads + tid;
+    int i = blockIdx.x * kScatterThreads + tid;
+    int32_t st_lo = -v.steps, st_hi = v.steps;  // the lane's samples
+    if constexpr (SPLIT) {
+        constexpr int kWaves = kScatterThreads / 64;
+        const int n_samples = 2 * v.steps + 1;
+        const int per_group = (n_samples + kWaves - 1) / kWaves;  // workgroups per point group
+        const int q = (int)(blockIdx.x % (uint32_t)per_group) * kWaves + (tid >> 6);
+        i = (int)(blockIdx.x / (uint32_t)per_group) * 64 + (tid & 63);
+        st_lo = q - v.steps;
+        st_hi = q < n_samples ? st_lo : st_lo - 1;  // wave-uniform: spare waves skip
+    }
     float px = 0.f, py = 0.f, pz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, d2 = 0.f, dist = 0.f;
     bool ok = i < src.n;
     if (ok) {
@@ -1675,7 +1692,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter_visit(VolDev v, Poi
         const int lane = tid & 63;
         int64_t last_key = kEmptyKey;
         int32_t last_bi = kIdxAbsent;
-        for (int32_t st = -v.steps; st <= v.steps; ++st) {
+        for (int32_t st = st_lo; st <= st_hi; ++st) {
             const float t = (float)st * v.voxel;
             const int32_t vx = (int32_t)floorf((px + ux * t) * v.inv_voxel);
             const int32_t vy = (int32_t)floorf((py + uy * t) * v.inv_voxel);
@@ -2006,23 +2023,33 @@ hipError_t launch_assign_batch(const BatchDev& b, uint32_t max_capacity, hipStre
 }
 
 hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, const ScatterScratch& sc,
-                                 bool dense, hipStream_t s) {
-    // the region of workgroup g starts at entry g * sc.region (32-bit)
-    const int64_t grid = ((int64_t)src.n + kScatterThreads - 1) / kScatterThreads;
-    if (src.n <= 0 || sc.region <= 0 || sc.region > kScatterSet ||
-        grid * sc.region > (int64_t)UINT32_MAX)
+                                 bool dense, bool split, hipStream_t s) {
+    // the region of workgroup g starts at entry g * sc.region (32-bit); it
+    // holds the workgroup's distinct voxels
+    const int64_t grid = scatter_visit_grid(src.n, v.steps, split);
+    const int32_t need = split ? scatter_region_split() : scatter_region(v.steps);
+    if (src.n <= 0 || sc.region < need || grid * sc.region > (int64_t)UINT32_MAX)
         return hipErrorInvalidValue;
     // voxel indices as 32-bit LDS keys while they fit (~0 stays free)
     const bool small_keys = (uint64_t)v.pool_capacity * kBlockVoxels < 0xFFFFFFFFull;
-    if (small_keys && dense)
-        k_scatter_visit<uint32_t, kScatterSetDense><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
-    else if (small_keys)
-        k_scatter_visit<uint32_t, kScatterSet><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
-    else if (dense)
-        k_scatter_visit<unsigned long long, kScatterSetDense>
+    if (split) {
+        if (small_keys)
+            k_scatter_visit<uint32_t, kScatterSetSplit, true><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+        else
+            k_scatter_visit<unsigned long long, kScatterSetSplit, true>
+                <<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    } else if (small_keys && dense) {
+        k_scatter_visit<uint32_t, kScatterSetDense, false>
             <<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
-    else
-        k_scatter_visit<unsigned long long, kScatterSet><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    } else if (small_keys) {
+        k_scatter_visit<uint32_t, kScatterSet, false><<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    } else if (dense) {
+        k_scatter_visit<unsigned long long, kScatterSetDense, false>
+            <<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    } else {
+        k_scatter_visit<unsigned long long, kScatterSet, false>
+            <<<(int)grid, kScatterThreads, 0, s>>>(v, src, sc);
+    }
     const int ag = v.pool_capacity < kScatterApplyGrid ? v.pool_capacity : kScatterApplyGrid;
     k_scatter_apply<<<ag, kScatterThreads, 0, s>>>(v, sc);
     return hipGetLastError();

@@ -217,7 +217,7 @@ struct OsnTsdfHipVolume {
     bool grow_stage(int64_t verts, bool rgb);
     bool grow_tiled(int32_t W, int32_t H);
     bool copy_to_host(void* dst, const void* src, size_t n);
-    bool grow_scatter(int32_t n_points);
+    bool grow_scatter(int64_t entries);
     // Wait for every stream that writes this volume: its own and, while it
     // belongs to a batch, the batch's (a non-blocking stream of its own).
     hipError_t sync() const {
@@ -430,7 +430,9 @@ bool OsnTsdfHipVolume::copy_to_host(void* dst, const void* src, size_t n) {
     return true;
 }
 
-bool OsnTsdfHipVolume::grow_scatter(int32_t n_points) {
+// The update's scratch: per-block parts at the first call, entries grown to
+// `entries` (a pass pair's visit workgroups times their region).
+bool OsnTsdfHipVolume::grow_scatter(int64_t entries) {
     if (!sc.seg_count) {
         const size_t pool = (size_t)cfg.pool_capacity_blocks;
         HIP_OK(hipMalloc(&sc.segs, pool * kScatterSegs * sizeof(ScatterSeg)));
@@ -441,7 +443,6 @@ bool OsnTsdfHipVolume::grow_scatter(int32_t n_points) {
         HIP_OK(hipMemsetAsync(sc.seg_count, 0, pool * sizeof(int32_t), stream));
         HIP_OK(hipMemsetAsync(sc.touched_count, 0, kScatterCtrInts * sizeof(int32_t), stream));
         sc.parity = 0;
-        sc.region = scatter_region(d.steps);
         // without the hint every call takes the larger set
         sc.touched_out = nullptr;
         if (hipHostMalloc((void**)&sc_touched_host, sizeof(int32_t), hipHostMallocMapped) ==
@@ -453,7 +454,7 @@ bool OsnTsdfHipVolume::grow_scatter(int32_t n_points) {
             sc_touched_host = nullptr;
         }
     }
-    const int64_t want = ((int64_t)n_points + kScatterThreads - 1) / kScatterThreads * sc.region;
+    const int64_t want = entries;
     if (want <= sc_entries_cap) return true;
     // the stream may still read the old buffer
     HIP_OK(hipStreamSynchronize(stream));
@@ -706,8 +707,16 @@ int32_t osn_tsdf_hip_update_voxels(OsnTsdfHipVolume* v, const float* d_positions
     // Chunks of points, each a call of the two-pass update: visits add up
     // and the cap takes against the weight each chunk finds, so the sums and
     // counts are those of one call over all points.
+    // Small calls take one sample per lane (kScatterSplitPoints < per).
     const int32_t per = v->scatter_chunk();
-    if (!v->grow_scatter(n_points < per ? n_points : per)) { v->ok = false; return 2; }
+    const bool split = n_points <= kScatterSplitPoints;
+    const int32_t region = split ? scatter_region_split() : scatter_region(v->d.steps);
+    const int32_t first = n_points < per ? n_points : per;
+    if (!v->grow_scatter(scatter_visit_grid(first, v->d.steps, split) * region)) {
+        v->ok = false;
+        return 2;
+    }
+    v->sc.region = region;
     hipEvent_t e;
     v->stage_begin(2, e);
     hipError_t err = hipSuccess;
@@ -718,7 +727,7 @@ int32_t osn_tsdf_hip_update_voxels(OsnTsdfHipVolume* v, const float* d_positions
         const bool dense = last > 0 && (int64_t)m >= (int64_t)kScatterDensePoints * last;
         err = launch_scatter_update(
             v->d, points_src(d_positions + (size_t)at * 3, m, cam_x, cam_y, cam_z, radius_m),
-            v->sc, dense, v->stream);
+            v->sc, dense, split, v->stream);
         v->sc.parity ^= 1;
     }
     v->stage_end(2, e);

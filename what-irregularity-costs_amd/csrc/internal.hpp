@@ -162,6 +162,11 @@ constexpr int kScatterThreads = 256;
 // half full.
 constexpr int kScatterSet = 3072;
 constexpr int kScatterSetDense = 2048;
+// Calls of at most kScatterSplitPoints points take one sample per lane (the
+// visit pass's SPLIT layout): a workgroup then holds kScatterThreads visits,
+// and its set kScatterSetSplit slots.
+constexpr int kScatterSplitPoints = 1 << 17;
+constexpr int kScatterSetSplit = 1024;
 // Points per touched block at which a call counts as dense.
 constexpr int kScatterDensePoints = 100;
 constexpr int kScatterBlocks = 256;  // distinct blocks per workgroup (one per thread)
@@ -185,6 +190,14 @@ __host__ __device__ inline int32_t scatter_region(int32_t steps) {
     const int64_t visits = (int64_t)kScatterThreads * (2 * (int64_t)steps + 1);
     return visits < kScatterSet ? (int32_t)visits : kScatterSet;
 }
+__host__ __device__ constexpr int32_t scatter_region_split() { return kScatterThreads; }
+// Visit-pass workgroups of a call of n points: one per kScatterThreads points,
+// or (split) ceil(samples / waves) per 64 points.
+__host__ __device__ inline int64_t scatter_visit_grid(int32_t n, int32_t steps, bool split) {
+    if (!split) return ((int64_t)n + kScatterThreads - 1) / kScatterThreads;
+    const int64_t waves = kScatterThreads / 64, samples = 2 * (int64_t)steps + 1;
+    return ((int64_t)n + 63) / 64 * ((samples + waves - 1) / waves);
+}
 
 struct ScatterScratch {
     uint2* entries;          // x = voxel in block | visits << 9, y = tsdf sum (float bits)
@@ -200,9 +213,10 @@ struct ScatterScratch {
     int32_t parity;
 };
 
-// dense: the visit pass takes the smaller LDS set (kScatterSetDense).
+// dense: the visit pass takes the smaller LDS set (kScatterSetDense);
+// split: one sample per lane (calls of at most kScatterSplitPoints points).
 hipError_t launch_scatter_update(const VolDev& v, const PointSrc& src, const ScatterScratch& sc,
-                                 bool dense, hipStream_t s);
+                                 bool dense, bool split, hipStream_t s);
 hipError_t launch_proj_update(const VolDev& v, const ProjCam& cam, CallInfo call, int32_t grid,
                               int32_t block_threads, hipStream_t s);
 // max_w_max: the largest weight cap of the batch's volumes (FP16 records
