@@ -229,6 +229,32 @@ def test_extract_bit_identical_to_oracle_extractor(H):
         assert np.array_equal(grgb, orgb)
 
 
+def test_host_extract_in_chunks_equals_device_extract(H):
+    """The host-buffer extract copies through two pinned 4 MiB halves on a
+    thread pool: the base sphere's 846 k vertices (20.3 MB, five chunks, an
+    uneven last one) and their colours must equal the device extraction byte
+    for byte, into an unaligned host buffer too."""
+    pts = B.sphere_points(0.5, 400, 800)
+    gv = H.HipVolume(0.01, 1 << 14, -1.0, 0.0)
+    assert gv.integrate(dev(pts), (0, 0, 0)) == 0
+    host, hrgb = gv.extract(0.5, 0.0, rgb=True)
+    n = len(host)
+    assert n * 24 > 4 * (4 << 20)
+    posnor = torch.empty(n * 6, dtype=torch.float32, device="cuda")
+    col = torch.empty(n * 3, dtype=torch.uint8, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    assert gv.extract_device(posnor, col, n, 0.5, 0.0, cnt) == 0
+    gv.synchronize()
+    assert int(cnt.item()) == n
+    assert np.array_equal(posnor.cpu().numpy().reshape(n, 6), host)
+    assert np.array_equal(col.cpu().numpy().reshape(n, 3), hrgb)
+    # a caller's buffer that starts 4 bytes into an allocation
+    raw = np.zeros(n * 6 + 1, np.float32)
+    got = gv._L.osn_tsdf_hip_extract_mesh(gv.h, raw[1:].ctypes.data, None, n, 0.5, 0.0)
+    assert got == n
+    assert np.array_equal(raw[1:].reshape(n, 6), host) and raw[0] == 0.0
+
+
 def test_extract_deterministic_and_overflow(H):
     pts = B.sphere_points(0.4, 200, 400)
     meshes = []
