@@ -1486,6 +1486,13 @@ __device__ void assign_volume(const VolDev& v, int parity, int wg, AssignLds& s)
     a.done = v.assign_done + (size_t)parity * assign_done_per_parity(v.capacity);
     a.bc0 = v.ctr->block_count;
     if (threadIdx.x == 0) s.drops = 0;
+    if (a.T == 1) {
+        // one tile, one item: workgroup 0 (the only one past the check
+        // above) takes it without a ticket
+        __syncthreads();
+        assign_item(v, a, s, 0, 0);
+        return;
+    }
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s.ticket = atomicAdd(&v.ctr->lists[parity].assign_ticket, 1);
